@@ -1,0 +1,174 @@
+// Fused Adam over every Gaussian tensor in one launch (gfx950).
+//
+// Reference: cuda/opt/adam.cu:6-84 runs one kernel over mean/rotation/scale/
+// opacity(/channels) per Gaussian and a second over the 16 SH coefficients,
+// then synchronises the device.  Here the seven tensors are streamed as flat
+// fp32 arrays in 16-byte vectors (each workgroup owns a contiguous slice of
+// one tensor, so every access is coalesced), and the pruning counter update of
+// gray/raytracer.py:361-366 rides along as an eighth segment.
+
+#include <math.h>
+
+#include "ctx.hpp"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kVecPerThread = 2;                         // float4 per thread
+constexpr int kFloatsPerBlock = kThreads * kVecPerThread * 4;  // 2048
+
+struct AdamArgs {
+    float *p[7], *g[7], *m[7], *v[7];
+    const float *lr[7];
+    int64_t numel[7];
+    int32_t block_start[9];  // first block of each segment (segment 7 = pruning counter)
+    const float *beta1, *beta2, *eps;
+    const uint8_t *visible;
+    int32_t *counter;
+    int32_t count;
+    int32_t host_flags;
+    int32_t step;
+    float step_plus_one;
+    const uint8_t *d_update_channels, *d_zero_grads, *d_enable_sh;
+    const int32_t *d_laziness;
+};
+
+// Segment mode from the host flags or the device-resident switches.
+__device__ __forceinline__ int seg_mode(const AdamArgs &A, int t, bool &zero) {
+    const bool upd_ch = A.d_update_channels ? *A.d_update_channels != 0 : (A.host_flags & 1) != 0;
+    zero = A.d_zero_grads ? *A.d_zero_grads != 0 : (A.host_flags & 2) != 0;
+    bool upd_sh;
+    if (A.d_enable_sh) {
+        const int lazy = A.d_laziness ? max(*A.d_laziness, 1) : 1;
+        upd_sh = *A.d_enable_sh != 0 && (A.step % lazy) == 0;
+    } else {
+        upd_sh = (A.host_flags & 4) != 0;
+    }
+    if (A.numel[t] <= 0) return 0;
+    if (t == 4) return upd_ch ? 1 : (zero ? 2 : 0);  // channels (adam.cu:35-40)
+    if (t >= 5) return upd_sh ? 1 : 0;               // SH coefficients (adam.cu:82-85)
+    return 1;
+}
+
+__device__ __forceinline__ void adam1(float &p, float &m, float &v, float &g, float lr, float b1, float b2, float bc1,
+                                      float bc2, float eps, bool zero) {
+    m = b1 * m + (1.0f - b1) * g;
+    v = b2 * v + (1.0f - b2) * g * g;
+    const float mh = m / bc1;
+    const float vh = v / bc2;
+    p -= lr * mh / (sqrtf(vh) + eps);
+    if (zero) g *= 0.0f;
+}
+
+__global__ __launch_bounds__(kThreads) void k_adam(AdamArgs A) {
+    const int b = blockIdx.x;
+    if (b >= A.block_start[7]) {
+        // pruning statistics: counter[i] += was_visible[i]
+        const int i = (b - A.block_start[7]) * kThreads + threadIdx.x;
+        if (A.visible && A.counter && i < A.count && A.visible[i]) A.counter[i] += 1;
+        return;
+    }
+    int t = 0;
+#pragma unroll
+    for (int k = 1; k < 7; ++k)
+        if (b >= A.block_start[k]) t = k;
+    bool zero;
+    const int mode = seg_mode(A, t, zero);
+    if (mode == 0) return;
+    const int64_t n = A.numel[t];
+    const int64_t base = (int64_t)(b - A.block_start[t]) * kFloatsPerBlock;
+    float *__restrict__ P = A.p[t];
+    float *__restrict__ G = A.g[t];
+    float *__restrict__ M = A.m[t];
+    float *__restrict__ V = A.v[t];
+    if (mode == 2) {
+        for (int j = 0; j < kVecPerThread * 4; ++j) {
+            const int64_t i = base + (int64_t)j * kThreads + threadIdx.x;
+            if (i < n) G[i] = G[i] * 0.0f;
+        }
+        return;
+    }
+    const float lr = *A.lr[t];
+    const float b1 = *A.beta1, b2 = *A.beta2, eps = *A.eps;
+    const float bc1 = 1.0f - powf(b1, A.step_plus_one);
+    const float bc2 = 1.0f - powf(b2, A.step_plus_one);
+#pragma unroll
+    for (int j = 0; j < kVecPerThread; ++j) {
+        const int64_t i = base + ((int64_t)j * kThreads + threadIdx.x) * 4;
+        if (i + 3 < n) {
+            float4 p = *reinterpret_cast<float4 *>(P + i);
+            float4 g = *reinterpret_cast<float4 *>(G + i);
+            float4 m = *reinterpret_cast<float4 *>(M + i);
+            float4 v = *reinterpret_cast<float4 *>(V + i);
+            adam1(p.x, m.x, v.x, g.x, lr, b1, b2, bc1, bc2, eps, zero);
+            adam1(p.y, m.y, v.y, g.y, lr, b1, b2, bc1, bc2, eps, zero);
+            adam1(p.z, m.z, v.z, g.z, lr, b1, b2, bc1, bc2, eps, zero);
+            adam1(p.w, m.w, v.w, g.w, lr, b1, b2, bc1, bc2, eps, zero);
+            *reinterpret_cast<float4 *>(P + i) = p;
+            *reinterpret_cast<float4 *>(M + i) = m;
+            *reinterpret_cast<float4 *>(V + i) = v;
+            if (zero) *reinterpret_cast<float4 *>(G + i) = g;
+        } else {
+            for (int64_t k = i; k < n; ++k) {
+                float p = P[k], g = G[k], m = M[k], v = V[k];
+                adam1(p, m, v, g, lr, b1, b2, bc1, bc2, eps, zero);
+                P[k] = p;
+                M[k] = m;
+                V[k] = v;
+                if (zero) G[k] = g;
+            }
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int gray_adam_step(gray_ctx *ctx, const gray_adam *adam, int32_t step, int32_t flags, void *stream) {
+    if (!ctx || !adam || !adam->beta1 || !adam->beta2 || !adam->epsilon) {
+        gray_set_error(GRAY_ERR_INVALID, "adam_step: null argument");
+        return GRAY_ERR_INVALID;
+    }
+    AdamArgs A{};
+    int32_t blocks = 0;
+    for (int t = 0; t < 7; ++t) {
+        A.p[t] = adam->param[t];
+        A.g[t] = adam->grad[t];
+        A.m[t] = adam->m1[t];
+        A.v[t] = adam->m2[t];
+        A.lr[t] = adam->lr[t];
+        A.numel[t] = adam->numel[t] > 0 ? adam->numel[t] : 0;
+        if (A.numel[t] > 0 && (!A.p[t] || !A.g[t] || !A.m[t] || !A.v[t] || !A.lr[t])) {
+            gray_set_error(GRAY_ERR_INVALID, "adam_step: tensor %d has a null pointer", t);
+            return GRAY_ERR_INVALID;
+        }
+        const float *ptrs[4] = {A.p[t], A.g[t], A.m[t], A.v[t]};
+        for (int k = 0; k < 4 && A.numel[t] > 0; ++k) {
+            if (((uintptr_t)ptrs[k]) & 15u) {
+                gray_set_error(GRAY_ERR_INVALID, "adam_step: tensor %d is not 16-byte aligned", t);
+                return GRAY_ERR_INVALID;
+            }
+        }
+        A.block_start[t] = blocks;
+        blocks += (int32_t)gray_div_up(A.numel[t], kFloatsPerBlock);
+    }
+    A.block_start[7] = blocks;
+    A.visible = adam->was_visible;
+    A.counter = adam->pruning_counter;
+    A.count = adam->count;
+    if (A.visible && A.counter && A.count > 0) blocks += (int32_t)gray_div_up(A.count, kThreads);
+    A.block_start[8] = blocks;
+    A.beta1 = adam->beta1;
+    A.beta2 = adam->beta2;
+    A.eps = adam->epsilon;
+    A.host_flags = flags;
+    A.step = step;
+    A.step_plus_one = (float)step + 1.0f;
+    A.d_update_channels = adam->update_channels;
+    A.d_zero_grads = adam->zero_grads;
+    A.d_enable_sh = adam->enable_sh;
+    A.d_laziness = adam->sh_update_laziness;
+    if (blocks == 0) return GRAY_OK;
+    hipLaunchKernelGGL(k_adam, dim3(blocks), dim3(kThreads), 0, (hipStream_t)stream, A);
+    GRAY_HIP(hipGetLastError());
+    return GRAY_OK;
+}

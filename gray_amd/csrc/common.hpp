@@ -1,0 +1,163 @@
+// Shared device/host definitions of the gfx950 Gaussian ray tracer.
+//
+// The per-(ray, Gaussian) hit criterion below is the contract that the CPU
+// oracle (oracle/gray_oracle.c) restates operation by operation: it uses only
+// IEEE-754 single-precision + - * / in a fixed order with FMA contraction
+// disabled, so a given (ray, snapshot) pair yields bit-identical hit sets,
+// distances and order on the GPU and on the CPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gray_hip.h"
+
+#define GRAY_WAVE 64
+#define GRAY_TILE 8             // 8x8 rays per wave64
+#define GRAY_KB GRAY_BATCH      // k-buffer entries per ray (reference BUFFER_SIZE)
+#define GRAY_STACK 128          // traversal stack entries per wave (LDS)
+#define GRAY_MAX_VIEWS 8        // views per launch (kernel-argument table)
+#define GRAY_LEAF 0x80000000u   // child reference flag: leaf, payload = Gaussian id
+#define GRAY_EMPTY 0xFFFFFFFFu  // child reference: nothing
+
+// Per-Gaussian geometry snapshot (64 B, read with one scalar load per leaf).
+// m = world-to-local linear part W2L = diag(1/(s*clip*gsf)) * R^T, row-major;
+// op = sigmoid(opacity) (or -1 when invisible), clip = adaptive clamp factor
+// (reference kernel.cu:3-22).
+struct __attribute__((aligned(16))) GeoRecord {
+    float m[9];
+    float mu[3];
+    float op;
+    float clip;
+    float pad0, pad1;
+};
+
+// BVH2 internal node with both child boxes inline (64 B, one scalar load).
+struct __attribute__((aligned(16))) BvhNode {
+    float lb[6];  // left child box: min xyz, max xyz
+    float rb[6];  // right child box
+    uint32_t left, right;  // child refs: internal index, or GRAY_LEAF|gaussian id
+    uint32_t pad0, pad1;
+};
+
+// Device copy of gray_knobs (device pointers to 1-element tensors).
+struct DevKnobs {
+    const float *exp_power, *alpha_threshold, *t_threshold, *gsf, *eps_fwd, *eps_scale_grad, *bg;
+    const uint8_t *render_depth, *enable_sh;
+};
+
+struct DevView {
+    const float *origin, *c2w, *fov, *znear, *zfar, *rays_o, *rays_d, *rgb;
+    const uint8_t *rays_en;
+    float *out_c, *out_depth, *out_T, *out_F, *out_est, *out_ro, *out_rd;
+    int32_t *stat_hits, *stat_acc;
+    uint32_t *hit_ids;
+    float *hit_T;
+    uint32_t *n_comp;
+    float *comp_rgb;
+    const float *grad_out;
+    float *grad_rgb;
+    int32_t width, height, stride, alloc_height;
+    int32_t tiles_x, tile_base, ray_count, pad;
+};
+
+struct TraceArgs {
+    DevKnobs knobs;
+    DevView views[GRAY_MAX_VIEWS];
+    int32_t num_views, total_tiles, save, capacity;
+    int32_t use_bvh, num_gaussians, xcd_chunk, pad;
+    const GeoRecord *geo;
+    const BvhNode *nodes;
+    const float *l2w;          // [N,12] local-to-world 3x3 (row-major, padded) of the snapshot
+    // backward-only
+    const float *rotation, *scale, *opacity;  // live parameters
+    float *g_mean, *g_rot, *g_scale, *g_op, *g_prune;
+    uint8_t *visible;
+    // debug hit lists
+    uint32_t *dbg_ids;
+    float *dbg_dist;
+    int32_t *dbg_count;
+    int32_t dbg_max, pad2;
+    int32_t *error_flag;       // set to a GRAY_ERR_* code on device-side failure
+};
+
+// ---------------------------------------------------------------------------
+// Exact ray / ellipsoid-proxy test (reference shaders.cu:32-47).
+//
+// (o, d) world ray; m, mu snapshot.  o' = W2L (o - mu), d' = W2L d.  Reject if
+// o'.d' > 0 (closest approach behind the origin).  The closest approach of the
+// line to the proxy centre is at world parameter t = -(o'.d')/(d'.d') (the
+// reference's t_local/|d'| with t_local = -o'.(d'/|d'|)), local point
+// x = o' + t d'.  Reject if |x|^2 > 1.  Returns t (the reference "distance").
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool gray_hit_test(const float *__restrict__ m, float mux, float muy, float muz,
+                                              float ox, float oy, float oz, float dx, float dy, float dz,
+                                              float &dist, float &hx, float &hy, float &hz) {
+#pragma clang fp contract(off)
+    const float px = ox - mux, py = oy - muy, pz = oz - muz;
+    const float lox = (m[0] * px + m[1] * py) + m[2] * pz;
+    const float loy = (m[3] * px + m[4] * py) + m[5] * pz;
+    const float loz = (m[6] * px + m[7] * py) + m[8] * pz;
+    const float ldx = (m[0] * dx + m[1] * dy) + m[2] * dz;
+    const float ldy = (m[3] * dx + m[4] * dy) + m[5] * dz;
+    const float ldz = (m[6] * dx + m[7] * dy) + m[8] * dz;
+    const float od = (lox * ldx + loy * ldy) + loz * ldz;
+    if (od > 0.0f) return false;
+    const float dd = (ldx * ldx + ldy * ldy) + ldz * ldz;
+    const float t = (-od) / dd;
+    hx = lox + t * ldx;
+    hy = loy + t * ldy;
+    hz = loz + t * ldz;
+    const float sq = (hx * hx + hy * hy) + hz * hz;
+    if (sq > 1.0f) return false;
+    dist = t;
+    return true;
+}
+
+// Generalised Gaussian kernel (reference kernel.cu:24-38) evaluated at the
+// local hit scaled by the clip factor; returns alpha (kernel.cu:40-42).
+// q_out = |x*clip|^2 and g_out = kernel value are returned for the backward.
+__device__ __forceinline__ float gray_alpha(float hx, float hy, float hz, float clip, float op, float p,
+                                            float &g_out, float &q_out) {
+#pragma clang fp contract(off)
+    const float lx = hx * clip, ly = hy * clip, lz = hz * clip;
+    const float q = (lx * lx + ly * ly) + lz * lz;
+    float g;
+    if (p == 1.0f) {
+        g = __expf(-0.5f * q);
+    } else if (p == 2.0f) {
+        g = __expf((-0.25f * q) * q);
+    } else if (p == 3.0f) {
+        g = __expf(-((q * q) * q) / 6.0f);
+    } else {
+        g = __expf(-__powf(q, p) / (2.0f * p));
+    }
+    g_out = g;
+    q_out = q;
+    return (GRAY_MAX_ALPHA * g) * op;
+}
+
+// Adaptive clamp factor (reference kernel.cu:3-22).
+__device__ __forceinline__ float gray_clip_factor(float op, float athr, float p) {
+    if (op <= athr) return 0.0f;
+    const float lr = logf(op / athr);
+    if (p == 1.0f) return sqrtf(2.0f * lr);
+    if (p == 2.0f) return sqrtf(sqrtf(4.0f * lr));
+    if (p == 3.0f) return cbrtf(sqrtf(6.0f * lr));
+    const float k = 2.0f * p;
+    return powf(k * lr, 1.0f / k);
+}
+
+__device__ __forceinline__ float gray_sigmoid(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+// Host-side error reporting shared by all translation units.
+void gray_set_error(int code, const char *fmt, ...);
+int gray_check_hip(hipError_t e, const char *what);
+
+#define GRAY_HIP(call)                                                   \
+    do {                                                                 \
+        hipError_t _e = (call);                                          \
+        if (_e != hipSuccess) return gray_check_hip(_e, #call);          \
+    } while (0)
+
+static inline unsigned gray_div_up(long long a, long long b) { return (unsigned)((a + b - 1) / b); }

@@ -1,0 +1,47 @@
+// Library context: owns the geometry snapshot, the LBVH and its build scratch.
+#pragma once
+
+#include "common.hpp"
+
+#define GRAY_CHUNK 1024  // leaves per refit chunk (phase-1 workgroup)
+
+struct gray_ctx {
+    int device = 0;
+    int64_t n = 0;         // Gaussians in the current snapshot
+    int64_t cap = 0;       // allocated capacity (Gaussians)
+    bool built = false;    // a topology exists for n
+    // geometry snapshot (original Gaussian order)
+    GeoRecord *geo = nullptr;
+    float *l2w = nullptr;   // [cap,12]
+    float *aabb = nullptr;  // [cap,6]
+    // LBVH (Karras layout: internal nodes 0..n-2, root 0)
+    BvhNode *nodes = nullptr;       // [max(cap-1,1)]
+    uint32_t *parent_int = nullptr; // [cap] parent | side<<31 (root: GRAY_EMPTY)
+    uint32_t *parent_leaf = nullptr;// [cap]
+    uint32_t *range_lo = nullptr;   // [cap]
+    uint32_t *range_hi = nullptr;   // [cap]
+    uint32_t *sorted_ids = nullptr; // [cap] leaf k -> Gaussian id
+    uint32_t *keys = nullptr;       // [cap] sorted Morton codes
+    // radix sort scratch
+    uint32_t *keys_alt = nullptr, *vals_alt = nullptr;
+    uint32_t *hist = nullptr;       // [256 * tiles]
+    uint32_t *block_sums = nullptr;
+    int64_t hist_cap = 0;
+    float *bounds_partial = nullptr;// [blocks*6]
+    float *bounds = nullptr;        // [6]
+    // refit scratch
+    float *prefix = nullptr, *suffix = nullptr; // [cap,6]
+    float *sparse = nullptr;        // [levels, chunks, 6]
+    int64_t sparse_cap = 0;
+    uint32_t *top_nodes = nullptr;  // [cap]
+    uint32_t *top_count = nullptr;  // [1] device
+    int64_t num_top = 0;
+    int32_t *err = nullptr;         // [1] device error flag
+    // timing
+    bool timing = false;
+    float last_build_ms = -1.0f;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+int gray_ctx_reserve(gray_ctx *ctx, int64_t n);
+int gray_refit(gray_ctx *ctx, hipStream_t s);

@@ -1,0 +1,913 @@
+// Wavefront-coherent Gaussian ray tracing on gfx950: forward, backward, debug.
+//
+// One workgroup = one wave64 = one 8x8 tile of rays of one view.  The wave
+// walks the LBVH as a packet: node and Gaussian records are fetched once per
+// wave with scalar loads (their addresses are wave-uniform), every lane runs
+// the slab / exact ellipsoid test for its own ray, and a ballot decides the
+// descent.  Every lane keeps its GRAY_KB nearest hits (distance, id, alpha)
+// sorted in LDS; compositing follows the reference batch semantics
+// (forward_pass.cu:150-230): each round takes the 32 nearest hits strictly
+// beyond tmin, composites front to back and stops at T < t_threshold.
+//
+// Truncation bookkeeping.  The reference estimates the colour of the hits it
+// never composited as (sum_all a*c - sum_comp a*c) / (sum_all a - sum_comp a)
+// and the full transmittance as prod_all (1-a) (forward_pass.cu:53-57,
+// 206-208, 238-240; shaders.cu:54-57).  The same quantities are formed here
+// directly over the non-composited set ("skip set"): a hit enters it when the
+// k-buffer rejects or evicts it, or when compositing stops before it; later
+// rounds take composited hits back out.  F = T * prod_skip (1-a).  This is the
+// same mathematics without the fp32 cancellation of sum_all - sum_comp.
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "ctx.hpp"
+
+namespace {
+
+struct LaneRay {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+};
+
+__device__ __forceinline__ bool slab_test(const LaneRay &r, const float *b, float tlo) {
+    const float tx0 = (b[0] - r.ox) * r.ix, tx1 = (b[3] - r.ox) * r.ix;
+    const float ty0 = (b[1] - r.oy) * r.iy, ty1 = (b[4] - r.oy) * r.iy;
+    const float tz0 = (b[2] - r.oz) * r.iz, tz1 = (b[5] - r.oz) * r.iz;
+    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    // relative slack keeps the cull conservative under rounding
+    const float slack = 1e-5f * fmaxf(fabsf(tn), fabsf(tf)) + 1e-7f;
+    return (b[0] <= b[3]) && (tn <= tf + slack) && (tf + slack >= tlo);
+}
+
+// XCD-aware block -> tile mapping: workgroup b runs on XCD (b % 8); give each
+// XCD a contiguous band of tiles so neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ bool block_tile(const TraceArgs &A, int &view, int &tx, int &ty) {
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * A.xcd_chunk + (b >> 3);
+    if (tile >= A.total_tiles) return false;
+    int v = 0;
+#pragma unroll
+    for (int k = 1; k < GRAY_MAX_VIEWS; ++k)
+        if (k < A.num_views && tile >= A.views[k].tile_base) v = k;
+    view = v;
+    const int lt = tile - A.views[v].tile_base;
+    tx = lt % A.views[v].tiles_x;
+    ty = lt / A.views[v].tiles_x;
+    return true;
+}
+
+// Primary ray of pixel (x, y): reference Camera::compute_primary_ray_direction
+// (camera.h:17-36) without jitter; explicit rays when the view provides them
+// (shaders.cu:80-85).  Returns false for an inactive pixel (zero direction).
+__device__ __forceinline__ bool make_ray(const DevView &V, int x, int y, LaneRay &r) {
+#pragma clang fp contract(off)
+    const int pix = y * V.stride + x;
+    float dx, dy, dz;
+    if (V.rays_d && (!V.rays_en || *V.rays_en)) {
+        r.ox = V.rays_o[3 * pix + 0];
+        r.oy = V.rays_o[3 * pix + 1];
+        r.oz = V.rays_o[3 * pix + 2];
+        dx = V.rays_d[3 * pix + 0];
+        dy = V.rays_d[3 * pix + 1];
+        dz = V.rays_d[3 * pix + 2];
+    } else {
+        r.ox = V.origin[0];
+        r.oy = V.origin[1];
+        r.oz = V.origin[2];
+        const float vs = tanf(*V.fov / 2.0f);
+        const float aspect = (float)V.width / (float)V.height;
+        const float yf = vs * (1.0f - (2.0f * ((float)y + 0.5f)) / (float)V.height);
+        const float xf = (aspect * vs) * ((2.0f * ((float)x + 0.5f)) / (float)V.width - 1.0f);
+        const float *c = V.c2w;  // row-major; direction = col0*x + col1*y - col2
+        dx = (c[0] * xf + c[1] * yf) - c[2];
+        dy = (c[3] * xf + c[4] * yf) - c[5];
+        dz = (c[6] * xf + c[7] * yf) - c[8];
+        const float len = sqrtf((dx * dx + dy * dy) + dz * dz);
+        dx = dx / len;
+        dy = dy / len;
+        dz = dz / len;
+    }
+    r.dx = dx;
+    r.dy = dy;
+    r.dz = dz;
+    r.ix = 1.0f / dx;
+    r.iy = 1.0f / dy;
+    r.iz = 1.0f / dz;
+    const float l2 = (dx * dx + dy * dy) + dz * dz;
+    return l2 != 0.0f;
+}
+
+// Packet traversal.  Calls leaf(gid) (wave-uniform gid) for every Gaussian
+// whose box some participating lane's ray interval [tlo, inf) overlaps, or for
+// every Gaussian when use_bvh == 0.  Must be called by the whole wave.
+template <typename Leaf>
+__device__ __forceinline__ void traverse(const TraceArgs &A, const BvhNode *__restrict__ nodes, const LaneRay &r,
+                                         bool part, float tlo, uint32_t *__restrict__ stack, float cdx, float cdy,
+                                         float cdz, Leaf &&leaf) {
+    const int n = A.num_gaussians;
+    if (n <= 0) return;
+    if (!A.use_bvh) {
+        for (int g = 0; g < n; ++g) leaf((uint32_t)g);
+        return;
+    }
+    uint32_t node = 0;
+    int sp = 0;
+    while (true) {
+        const BvhNode nd = nodes[node];
+        const bool hl = part && nd.left != GRAY_EMPTY && slab_test(r, nd.lb, tlo);
+        const bool hr = part && nd.right != GRAY_EMPTY && slab_test(r, nd.rb, tlo);
+        const uint64_t ml = __ballot(hl), mr = __ballot(hr);
+        uint32_t next = GRAY_EMPTY, other = GRAY_EMPTY;
+        if (ml) {
+            if (nd.left & GRAY_LEAF)
+                leaf(nd.left & ~GRAY_LEAF);
+            else
+                next = nd.left;
+        }
+        if (mr) {
+            if (nd.right & GRAY_LEAF)
+                leaf(nd.right & ~GRAY_LEAF);
+            else if (next == GRAY_EMPTY)
+                next = nd.right;
+            else
+                other = nd.right;
+        }
+        if (other != GRAY_EMPTY) {
+            // near child first along the tile's central direction
+            const float cl = ((nd.lb[0] + nd.lb[3]) * cdx + (nd.lb[1] + nd.lb[4]) * cdy) + (nd.lb[2] + nd.lb[5]) * cdz;
+            const float cr = ((nd.rb[0] + nd.rb[3]) * cdx + (nd.rb[1] + nd.rb[4]) * cdy) + (nd.rb[2] + nd.rb[5]) * cdz;
+            if (__builtin_amdgcn_readfirstlane((int)(cr < cl))) {
+                const uint32_t t = next;
+                next = other;
+                other = t;
+            }
+            if (sp >= GRAY_STACK) {
+                if (A.error_flag) atomicOr(A.error_flag, 1 << GRAY_ERR_OVERFLOW);
+                return;
+            }
+            stack[sp] = other;
+            ++sp;
+        }
+        if (next == GRAY_EMPTY) {
+            if (sp == 0) break;
+            --sp;
+            node = (uint32_t)__builtin_amdgcn_readfirstlane((int)stack[sp]);
+        } else {
+            node = next;
+        }
+    }
+}
+
+// Sorted per-lane k-buffer in LDS, layout [slot][lane].
+struct KBuf {
+    float *d;
+    uint32_t *id;
+    float *a;
+    int lane;
+    __device__ __forceinline__ float dist(int k) const { return d[k * GRAY_WAVE + lane]; }
+    __device__ __forceinline__ uint32_t gid(int k) const { return id[k * GRAY_WAVE + lane]; }
+    __device__ __forceinline__ float alpha(int k) const { return a[k * GRAY_WAVE + lane]; }
+};
+
+__device__ __forceinline__ bool key_less(float d0, uint32_t i0, float d1, uint32_t i1) {
+    return d0 < d1 || (d0 == d1 && i0 < i1);
+}
+
+// Insert (dist, id, alpha).  Returns 0 = rejected, 1 = inserted, 2 = inserted
+// and the previous last entry (ev_id, ev_a) was evicted.
+__device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &lastd, uint32_t &lasti, float d,
+                                         uint32_t gid, float a, uint32_t &ev_id, float &ev_a) {
+    int pos;
+    int res = 1;
+    if (count == GRAY_KB) {
+        if (!key_less(d, gid, lastd, lasti)) return 0;
+        ev_id = kb.gid(GRAY_KB - 1);
+        ev_a = kb.alpha(GRAY_KB - 1);
+        pos = GRAY_KB - 1;
+        res = 2;
+    } else {
+        pos = count;
+        ++count;
+    }
+    while (pos > 0) {
+        const float pd = kb.dist(pos - 1);
+        const uint32_t pi = kb.gid(pos - 1);
+        if (!key_less(d, gid, pd, pi)) break;
+        kb.d[pos * GRAY_WAVE + kb.lane] = pd;
+        kb.id[pos * GRAY_WAVE + kb.lane] = pi;
+        kb.a[pos * GRAY_WAVE + kb.lane] = kb.alpha(pos - 1);
+        --pos;
+    }
+    kb.d[pos * GRAY_WAVE + kb.lane] = d;
+    kb.id[pos * GRAY_WAVE + kb.lane] = gid;
+    kb.a[pos * GRAY_WAVE + kb.lane] = a;
+    lastd = kb.dist(count - 1);
+    lasti = kb.gid(count - 1);
+    return res;
+}
+
+__device__ __forceinline__ void tile_central_dir(const DevView &V, int tx, int ty, const LaneRay &r, bool active,
+                                                 float &cx, float &cy, float &cz) {
+    // direction of the first active lane (all rays of a tile are close)
+    const uint64_t m = __ballot(active);
+    if (!m) {
+        cx = cy = cz = 0.0f;
+        return;
+    }
+    const int src = __ffsll((long long)m) - 1;
+    cx = __shfl(r.dx, src);
+    cy = __shfl(r.dy, src);
+    cz = __shfl(r.dz, src);
+    cx = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cx)));
+    cy = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cy)));
+    cz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cz)));
+}
+
+// ---------------------------------------------------------------------------
+// Forward
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const GeoRecord *__restrict__ geo,
+                                                             const BvhNode *__restrict__ nodes,
+                                                             const float *__restrict__ rgb0) {
+    __shared__ float s_kd[GRAY_KB * GRAY_WAVE];
+    __shared__ uint32_t s_ki[GRAY_KB * GRAY_WAVE];
+    __shared__ float s_ka[GRAY_KB * GRAY_WAVE];
+    __shared__ uint32_t s_stack[GRAY_STACK];
+
+    int vi, tx, ty;
+    if (!block_tile(A, vi, tx, ty)) return;
+    const DevView &V = A.views[vi];
+    const int lane = threadIdx.x;
+    const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
+    const bool inside = x < V.width && y < V.height;
+    LaneRay r{};
+    bool active = false;
+    if (inside) active = make_ray(V, x, y, r);
+    float cdx, cdy, cdz;
+    tile_central_dir(V, tx, ty, r, active, cdx, cdy, cdz);
+
+    const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power, tthr = *A.knobs.t_threshold;
+    const float znear = *V.znear, zfar = *V.zfar;
+    const bool depth_on = *A.knobs.render_depth != 0;
+    const int pix = y * V.stride + x;
+    const int R = V.ray_count;
+    const float *__restrict__ rgb = rgb0 + (V.rgb - A.views[0].rgb);
+
+    KBuf kb{s_kd, s_ki, s_ka, lane};
+    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
+    float skc0 = 0.0f, skc1 = 0.0f, skc2 = 0.0f, ska = 0.0f, pnc = 1.0f;
+    float tmin = znear;
+    uint32_t ncomp = 0, nhits = 0;
+    bool alive = active;
+
+    for (int round = 0; round < GRAY_MAX_ROUNDS; ++round) {
+        const bool part = alive && tmin < zfar;
+        if (!__ballot(part)) break;
+        int count = 0;
+        float lastd = 0.0f;
+        uint32_t lasti = 0;
+        const bool first = round == 0;
+        auto leaf = [&](uint32_t gid) {
+            const GeoRecord rec = geo[gid];
+            if (rec.op <= athr) return;
+            float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+            if (first) {
+                c0 = rgb[3 * gid + 0];
+                c1 = rgb[3 * gid + 1];
+                c2 = rgb[3 * gid + 2];
+            }
+            if (!part) return;
+            float dist, hx, hy, hz;
+            if (!gray_hit_test(rec.m, rec.mu[0], rec.mu[1], rec.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx,
+                               hy, hz))
+                return;
+            float gv, q;
+            const float a = gray_alpha(hx, hy, hz, rec.clip, rec.op, p, gv, q);
+            if (first) ++nhits;
+            int res = 0;
+            uint32_t ev_id = 0;
+            float ev_a = 0.0f;
+            if (dist > tmin) res = kb_insert(kb, count, lastd, lasti, dist, gid, a, ev_id, ev_a);
+            if (!first) return;
+            if (res == 0) {
+                skc0 += a * c0;
+                skc1 += a * c1;
+                skc2 += a * c2;
+                ska += a;
+                pnc = pnc * (1.0f - a);
+            } else if (res == 2) {
+                skc0 += ev_a * rgb[3 * ev_id + 0];
+                skc1 += ev_a * rgb[3 * ev_id + 1];
+                skc2 += ev_a * rgb[3 * ev_id + 2];
+                ska += ev_a;
+                pnc = pnc * (1.0f - ev_a);
+            }
+        };
+        traverse(A, nodes, r, part, first ? 0.0f : tmin, s_stack, cdx, cdy, cdz, leaf);
+
+        if (part) {
+            bool terminated = false;
+            for (int k = 0; k < count; ++k) {
+                const float d = kb.dist(k);
+                const uint32_t gid = kb.gid(k);
+                const float a = kb.alpha(k);
+                if (terminated) {
+                    if (first) {
+                        skc0 += a * rgb[3 * gid + 0];
+                        skc1 += a * rgb[3 * gid + 1];
+                        skc2 += a * rgb[3 * gid + 2];
+                        ska += a;
+                        pnc = pnc * (1.0f - a);
+                    }
+                    continue;
+                }
+                tmin = fmaxf(d, tmin);
+                const float c0 = rgb[3 * gid + 0], c1 = rgb[3 * gid + 1], c2 = rgb[3 * gid + 2];
+                if (d < zfar) {
+                    if (!first) {
+                        skc0 -= a * c0;
+                        skc1 -= a * c1;
+                        skc2 -= a * c2;
+                        ska -= a;
+                        pnc = pnc / (1.0f - a);
+                    }
+                    const float w = T * a;
+                    C0 += c0 * w;
+                    C1 += c1 * w;
+                    C2 += c2 * w;
+                    if (depth_on) D += d * w;
+                    if (A.save && (int)ncomp < A.capacity) {
+                        V.hit_ids[(size_t)ncomp * R + pix] = gid;
+                        V.hit_T[(size_t)ncomp * R + pix] = T;
+                    }
+                    T = T * (1.0f - a);
+                    ++ncomp;
+                    if (T < tthr) {
+                        tmin = zfar;
+                        terminated = true;
+                    }
+                } else if (first) {
+                    skc0 += a * c0;
+                    skc1 += a * c1;
+                    skc2 += a * c2;
+                    ska += a;
+                    pnc = pnc * (1.0f - a);
+                }
+            }
+            if (!terminated && count < GRAY_KB) tmin = FLT_MAX;  // empty slots (reference: FLT_MAX)
+            alive = !terminated && count == GRAY_KB && tmin < zfar;
+        }
+    }
+
+    if (!active) return;
+    const float F = T * pnc;
+    const float rem = T - F;
+    float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+    if (ska > 0.0f) {
+        e0 = skc0 / ska;
+        e1 = skc1 / ska;
+        e2 = skc2 / ska;
+    }
+    const float cc0 = C0, cc1 = C1, cc2 = C2;
+    C0 += rem * e0;
+    C1 += rem * e1;
+    C2 += rem * e2;
+    if (depth_on) D = D / fmaxf(1.0f - T, *A.knobs.eps_fwd);
+    const float *bg = A.knobs.bg;
+    C0 += bg[0] * F;
+    C1 += bg[1] * F;
+    C2 += bg[2] * F;
+    V.out_c[3 * pix + 0] = C0;
+    V.out_c[3 * pix + 1] = C1;
+    V.out_c[3 * pix + 2] = C2;
+    if (V.out_depth) V.out_depth[pix] = D;
+    if (A.save) {
+        if (V.out_T) V.out_T[pix] = T;
+        if (V.out_F) V.out_F[pix] = F;
+        if (V.out_est) {
+            V.out_est[3 * pix + 0] = e0;
+            V.out_est[3 * pix + 1] = e1;
+            V.out_est[3 * pix + 2] = e2;
+        }
+        if (V.out_ro) {
+            V.out_ro[3 * pix + 0] = r.ox;
+            V.out_ro[3 * pix + 1] = r.oy;
+            V.out_ro[3 * pix + 2] = r.oz;
+        }
+        if (V.out_rd) {
+            V.out_rd[3 * pix + 0] = r.dx;
+            V.out_rd[3 * pix + 1] = r.dy;
+            V.out_rd[3 * pix + 2] = r.dz;
+        }
+        if (V.n_comp) V.n_comp[pix] = ncomp;
+        if (V.comp_rgb) {
+            V.comp_rgb[3 * pix + 0] = cc0;
+            V.comp_rgb[3 * pix + 1] = cc1;
+            V.comp_rgb[3 * pix + 2] = cc2;
+        }
+        if (V.stat_hits) V.stat_hits[pix] += (int)nhits;
+        if (V.stat_acc) V.stat_acc[pix] += (int)ncomp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backward: gradient of one composited hit (reference backward_pass.cu:15-141)
+// ---------------------------------------------------------------------------
+struct PixelState {
+    float g0, g1, g2;      // dL/d(out)
+    float Tf, F;           // final and full transmittance
+    float e0, e1, e2;      // truncation estimate
+    float bgdot, estdot;   // bg.g, est.g
+    float pixels;          // width*height of the launch (pruning weight normaliser)
+};
+
+__device__ __forceinline__ void backward_hit(const TraceArgs &A, const DevView &V, const LaneRay &r, float p,
+                                             float eps_sg, const PixelState &ps, uint32_t gid, float Ti, float &gT) {
+#pragma clang fp contract(off)
+    const GeoRecord rec = A.geo[gid];
+    float dist, hx, hy, hz;
+    if (!gray_hit_test(rec.m, rec.mu[0], rec.mu[1], rec.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz))
+        return;
+    float gv, q;
+    const float a = gray_alpha(hx, hy, hz, rec.clip, rec.op, p, gv, q);
+    const float c0 = V.rgb[3 * gid + 0], c1 = V.rgb[3 * gid + 1], c2 = V.rgb[3 * gid + 2];
+    const float w = Ti * a;
+    // colour gradient
+    atomicAdd(&V.grad_rgb[3 * gid + 0], ps.g0 * w);
+    atomicAdd(&V.grad_rgb[3 * gid + 1], ps.g1 * w);
+    atomicAdd(&V.grad_rgb[3 * gid + 2], ps.g2 * w);
+    // alpha gradient
+    const float gdot = (ps.g0 * c0 + ps.g1 * c1) + ps.g2 * c2;
+    const float oma = 1.0f - a;
+    float ga = (gdot - gT) * Ti;
+    ga += ((-ps.F) / oma) * ps.bgdot;
+    ga += -((ps.Tf - ps.F) / oma) * ps.estdot;
+    gT += (gdot - gT) * a;
+    // opacity (live parameters, reference helpers.cu:9-34)
+    const float op = gray_sigmoid(A.opacity[gid]);
+    float gop = (GRAY_MAX_ALPHA * ga) * gv;
+    gop = (gop * op) * (1.0f - op);
+    const float ggv = (GRAY_MAX_ALPHA * ga) * op;
+    // kernel -> local hit
+    const float clip = rec.clip;
+    const float lx = hx * clip, ly = hy * clip, lz = hz * clip;
+    float gsq;
+    if (p == 1.0f)
+        gsq = gv;
+    else if (p == 2.0f)
+        gsq = gv * q;
+    else if (p == 3.0f)
+        gsq = (gv * q) * q;
+    else
+        gsq = gv * powf(q, p - 1.0f);
+    const float gxl0 = (-lx * gsq) * ggv, gxl1 = (-ly * gsq) * ggv, gxl2 = (-lz * gsq) * ggv;
+    // local -> world hit point (W2L^T), times the clip factor
+    const float *m = rec.m;
+    const float gxw0 = ((m[0] * gxl0 + m[3] * gxl1) + m[6] * gxl2) * clip;
+    const float gxw1 = ((m[1] * gxl0 + m[4] * gxl1) + m[7] * gxl2) * clip;
+    const float gxw2 = ((m[2] * gxl0 + m[5] * gxl1) + m[8] * gxl2) * clip;
+    // d/d(L2W rows)
+    const float gl[3][3] = {{-gxw0 * lx, -gxw0 * ly, -gxw0 * lz},
+                            {-gxw1 * lx, -gxw1 * ly, -gxw1 * lz},
+                            {-gxw2 * lx, -gxw2 * ly, -gxw2 * lz}};
+    // scale
+    const float s0 = expf(A.scale[3 * gid + 0]), s1 = expf(A.scale[3 * gid + 1]), s2 = expf(A.scale[3 * gid + 2]);
+    const float *L = A.l2w + 12 * (size_t)gid;
+    const float d0 = s0 * clip + eps_sg, d1 = s1 * clip + eps_sg, d2 = s2 * clip + eps_sg;
+    const float gs0 = ((gl[0][0] * (L[0] / d0) + gl[1][0] * (L[3] / d0)) + gl[2][0] * (L[6] / d0)) * s0;
+    const float gs1 = ((gl[0][1] * (L[1] / d1) + gl[1][1] * (L[4] / d1)) + gl[2][1] * (L[7] / d1)) * s1;
+    const float gs2 = ((gl[0][2] * (L[2] / d2) + gl[1][2] * (L[5] / d2)) + gl[2][2] * (L[8] / d2)) * s2;
+    // rotation matrix rows
+    const float g0x = gl[0][0] * s0, g0y = gl[0][1] * s1, g0z = gl[0][2] * s2;
+    const float g1x = gl[1][0] * s0, g1y = gl[1][1] * s1, g1z = gl[1][2] * s2;
+    const float g2x = gl[2][0] * s0, g2y = gl[2][1] * s1, g2z = gl[2][2] * s2;
+    const float qr = A.rotation[4 * gid + 0], qx = A.rotation[4 * gid + 1], qy = A.rotation[4 * gid + 2],
+                qz = A.rotation[4 * gid + 3];
+    const float qlen = sqrtf(((qr * qr + qx * qx) + qy * qy) + qz * qz);
+    const float nr = qr / qlen, nx = qx / qlen, ny = qy / qlen, nz = qz / qlen;
+    const float gr = (2.f * nx * (g2y - g1z) + 2.f * ny * (g0z - g2x)) + 2.f * nz * (g1x - g0y);
+    const float gx = ((-4.f * nx * (g1y + g2z) + 2.f * ny * (g0y + g1x)) + 2.f * nz * (g0z + g2x)) + 2.f * nr * (g2y - g1z);
+    const float gy = ((2.f * nx * (g0y + g1x) - 4.f * ny * (g0x + g2z)) + 2.f * nz * (g1z + g2y)) + 2.f * nr * (g0z - g2x);
+    const float gz = ((2.f * nx * (g0z + g2x) + 2.f * ny * (g1z + g2y)) - 4.f * nz * (g0x + g1y)) + 2.f * nr * (g1x - g0y);
+    const float qdot = ((gr * qr + gx * qx) + gy * qy) + gz * qz;
+    const float len3 = powf(qlen, 3.0f);
+    const float q0 = (qdot * -qr) / len3 + gr / qlen;
+    const float q1 = (qdot * -qx) / len3 + gx / qlen;
+    const float q2 = (qdot * -qy) / len3 + gy / qlen;
+    const float q3 = (qdot * -qz) / len3 + gz / qlen;
+
+    atomicAdd(&A.g_rot[4 * gid + 0], q0);
+    atomicAdd(&A.g_rot[4 * gid + 1], q1);
+    atomicAdd(&A.g_rot[4 * gid + 2], q2);
+    atomicAdd(&A.g_rot[4 * gid + 3], q3);
+    atomicAdd(&A.g_scale[3 * gid + 0], gs0);
+    atomicAdd(&A.g_scale[3 * gid + 1], gs1);
+    atomicAdd(&A.g_scale[3 * gid + 2], gs2);
+    atomicAdd(&A.g_mean[3 * gid + 0], -gxw0);
+    atomicAdd(&A.g_mean[3 * gid + 1], -gxw1);
+    atomicAdd(&A.g_mean[3 * gid + 2], -gxw2);
+    atomicAdd(&A.g_op[gid], gop);
+    if (A.g_prune) atomicAdd(&A.g_prune[gid], w / ps.pixels);
+    if (A.visible) A.visible[gid] = 1;
+}
+
+__device__ __forceinline__ bool load_pixel_state(const TraceArgs &A, const DevView &V, int pix, LaneRay &r,
+                                                 PixelState &ps) {
+    r.ox = V.out_ro[3 * pix + 0];
+    r.oy = V.out_ro[3 * pix + 1];
+    r.oz = V.out_ro[3 * pix + 2];
+    r.dx = V.out_rd[3 * pix + 0];
+    r.dy = V.out_rd[3 * pix + 1];
+    r.dz = V.out_rd[3 * pix + 2];
+    r.ix = 1.0f / r.dx;
+    r.iy = 1.0f / r.dy;
+    r.iz = 1.0f / r.dz;
+    ps.g0 = V.grad_out[3 * pix + 0];
+    ps.g1 = V.grad_out[3 * pix + 1];
+    ps.g2 = V.grad_out[3 * pix + 2];
+    ps.Tf = V.out_T[pix];
+    ps.F = V.out_F[pix];
+    ps.e0 = V.out_est[3 * pix + 0];
+    ps.e1 = V.out_est[3 * pix + 1];
+    ps.e2 = V.out_est[3 * pix + 2];
+    const float *bg = A.knobs.bg;
+    ps.bgdot = (bg[0] * ps.g0 + bg[1] * ps.g1) + bg[2] * ps.g2;
+    ps.estdot = (ps.e0 * ps.g0 + ps.e1 * ps.g1) + ps.e2 * ps.g2;
+    ps.pixels = (float)(V.width * V.height);
+    return ((r.dx * r.dx + r.dy * r.dy) + r.dz * r.dz) != 0.0f;
+}
+
+// Rays whose composited list fits the store: replay it back to front.
+__global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored(TraceArgs A) {
+    int vi, tx, ty;
+    if (!block_tile(A, vi, tx, ty)) return;
+    const DevView &V = A.views[vi];
+    const int lane = threadIdx.x;
+    const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
+    if (x >= V.width || y >= V.height) return;
+    const int pix = y * V.stride + x;
+    const int n = (int)V.n_comp[pix];
+    if (n == 0 || n > A.capacity) return;
+    LaneRay r;
+    PixelState ps;
+    if (!load_pixel_state(A, V, pix, r, ps)) return;
+    const float p = *A.knobs.exp_power, eps_sg = *A.knobs.eps_scale_grad;
+    const int R = V.ray_count;
+    float gT = 0.0f;
+    for (int k = n - 1; k >= 0; --k) {
+        const uint32_t gid = V.hit_ids[(size_t)k * R + pix];
+        const float Ti = V.hit_T[(size_t)k * R + pix];
+        backward_hit(A, V, r, p, eps_sg, ps, gid, Ti, gT);
+    }
+}
+
+// Rays whose list exceeded the store (or capacity 0): regenerate each round by
+// re-traversal, sweep it forward (T, partial sums) and back (gradients).  The
+// gradient flowing into a round from the rounds behind it is
+// (g.C_comp - sum_{composited so far} g.c w) / T.
+__global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArgs A, const GeoRecord *__restrict__ geo,
+                                                                        const BvhNode *__restrict__ nodes) {
+    __shared__ float s_kd[GRAY_KB * GRAY_WAVE];
+    __shared__ uint32_t s_ki[GRAY_KB * GRAY_WAVE];
+    __shared__ float s_ka[GRAY_KB * GRAY_WAVE];
+    __shared__ uint32_t s_stack[GRAY_STACK];
+    int vi, tx, ty;
+    if (!block_tile(A, vi, tx, ty)) return;
+    const DevView &V = A.views[vi];
+    const int lane = threadIdx.x;
+    const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
+    const bool inside = x < V.width && y < V.height;
+    const int pix = y * V.stride + x;
+    int n = 0;
+    if (inside) n = (int)V.n_comp[pix];
+    bool need = inside && n > A.capacity;
+    if (!__ballot(need)) return;
+    LaneRay r{};
+    PixelState ps{};
+    if (need) need = load_pixel_state(A, V, pix, r, ps);
+    float cdx, cdy, cdz;
+    tile_central_dir(V, tx, ty, r, need, cdx, cdy, cdz);
+    const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power, tthr = *A.knobs.t_threshold;
+    const float eps_sg = *A.knobs.eps_scale_grad;
+    const float znear = *V.znear, zfar = *V.zfar;
+    KBuf kb{s_kd, s_ki, s_ka, lane};
+    float Atot = 0.0f;
+    if (need)
+        Atot = (ps.g0 * V.comp_rgb[3 * pix + 0] + ps.g1 * V.comp_rgb[3 * pix + 1]) + ps.g2 * V.comp_rgb[3 * pix + 2];
+    float P = 0.0f, T = 1.0f, tmin = znear;
+    int done = 0;
+    bool alive = need;
+    for (int round = 0; round < GRAY_MAX_ROUNDS; ++round) {
+        const bool part = alive && tmin < zfar;
+        if (!__ballot(part)) break;
+        int count = 0;
+        float lastd = 0.0f;
+        uint32_t lasti = 0;
+        auto leaf = [&](uint32_t gid) {
+            const GeoRecord rec = geo[gid];
+            if (rec.op <= athr) return;
+            if (!part) return;
+            float dist, hx, hy, hz;
+            if (!gray_hit_test(rec.m, rec.mu[0], rec.mu[1], rec.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx,
+                               hy, hz))
+                return;
+            if (!(dist > tmin)) return;
+            float gv, q;
+            const float a = gray_alpha(hx, hy, hz, rec.clip, rec.op, p, gv, q);
+            uint32_t ev_id;
+            float ev_a;
+            kb_insert(kb, count, lastd, lasti, dist, gid, a, ev_id, ev_a);
+        };
+        traverse(A, nodes, r, part, tmin, s_stack, cdx, cdy, cdz, leaf);
+        if (part) {
+            bool terminated = false;
+            int used = 0;
+            for (int k = 0; k < count; ++k) {
+                const float d = kb.dist(k);
+                const uint32_t gid = kb.gid(k);
+                const float a = kb.alpha(k);
+                if (terminated) {
+                    kb.id[k * GRAY_WAVE + lane] = GRAY_EMPTY;
+                    continue;
+                }
+                tmin = fmaxf(d, tmin);
+                if (d < zfar) {
+                    const float c0 = V.rgb[3 * gid + 0], c1 = V.rgb[3 * gid + 1], c2 = V.rgb[3 * gid + 2];
+                    const float w = T * a;
+                    P += ((ps.g0 * c0 + ps.g1 * c1) + ps.g2 * c2) * w;
+                    kb.a[k * GRAY_WAVE + lane] = T;  // T before this hit
+                    T = T * (1.0f - a);
+                    ++done;
+                    used = k + 1;
+                    if (T < tthr) {
+                        tmin = zfar;
+                        terminated = true;
+                    }
+                } else {
+                    kb.id[k * GRAY_WAVE + lane] = GRAY_EMPTY;
+                }
+            }
+            if (!terminated && count < GRAY_KB) tmin = FLT_MAX;
+            const bool last = done >= n;
+            float gT = last ? 0.0f : (Atot - P) / T;
+            for (int k = used - 1; k >= 0; --k) {
+                const uint32_t gid = kb.gid(k);
+                if (gid == GRAY_EMPTY) continue;
+                backward_hit(A, V, r, p, eps_sg, ps, gid, kb.alpha(k), gT);
+            }
+            alive = !last && !terminated && count == GRAY_KB && tmin < zfar;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Debug: every hit of every ray of view 0 (unsorted), then a per-ray sort.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(GRAY_WAVE) void k_debug_hits(TraceArgs A, const GeoRecord *__restrict__ geo,
+                                                          const BvhNode *__restrict__ nodes) {
+    __shared__ uint32_t s_stack[GRAY_STACK];
+    int vi, tx, ty;
+    if (!block_tile(A, vi, tx, ty)) return;
+    if (vi != 0) return;
+    const DevView &V = A.views[0];
+    const int lane = threadIdx.x;
+    const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
+    const bool inside = x < V.width && y < V.height;
+    LaneRay r{};
+    bool active = false;
+    if (inside) active = make_ray(V, x, y, r);
+    float cdx, cdy, cdz;
+    tile_central_dir(V, tx, ty, r, active, cdx, cdy, cdz);
+    const float athr = *A.knobs.alpha_threshold;
+    const int ray = y * V.width + x;
+    int cnt = 0;
+    auto leaf = [&](uint32_t gid) {
+        const GeoRecord rec = geo[gid];
+        if (rec.op <= athr) return;
+        if (!active) return;
+        float dist, hx, hy, hz;
+        if (!gray_hit_test(rec.m, rec.mu[0], rec.mu[1], rec.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy,
+                           hz))
+            return;
+        if (cnt < A.dbg_max) {
+            A.dbg_ids[(size_t)ray * A.dbg_max + cnt] = gid;
+            A.dbg_dist[(size_t)ray * A.dbg_max + cnt] = dist;
+        }
+        ++cnt;
+    };
+    traverse(A, nodes, r, active, 0.0f, s_stack, cdx, cdy, cdz, leaf);
+    if (inside) A.dbg_count[ray] = active ? cnt : 0;
+}
+
+__global__ void k_debug_sort(int rays, int maxh, uint32_t *__restrict__ ids, float *__restrict__ dist,
+                             const int32_t *__restrict__ counts) {
+    const int ray = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= rays) return;
+    const int n = min(counts[ray], maxh);
+    uint32_t *id = ids + (size_t)ray * maxh;
+    float *ds = dist + (size_t)ray * maxh;
+    for (int i = 1; i < n; ++i) {
+        const float d = ds[i];
+        const uint32_t g = id[i];
+        int j = i - 1;
+        while (j >= 0 && key_less(d, g, ds[j], id[j])) {
+            ds[j + 1] = ds[j];
+            id[j + 1] = id[j];
+            --j;
+        }
+        ds[j + 1] = d;
+        id[j + 1] = g;
+    }
+}
+
+int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int nviews, TraceArgs &A) {
+    memset(&A, 0, sizeof(A));
+    const gray_knobs &k = args->knobs;
+    if (!k.exp_power || !k.alpha_threshold || !k.t_threshold || !k.eps_forward_normalization || !k.eps_scale_grad ||
+        !k.background_channels || !k.render_depth) {
+        gray_set_error(GRAY_ERR_INVALID, "render: null knob pointer");
+        return GRAY_ERR_INVALID;
+    }
+    A.knobs = DevKnobs{k.exp_power, k.alpha_threshold, k.t_threshold, k.global_scale_factor,
+                       k.eps_forward_normalization, k.eps_scale_grad, k.background_channels, k.render_depth,
+                       k.enable_sh};
+    int tiles = 0;
+    for (int i = 0; i < nviews; ++i) {
+        const gray_view &v = args->views[first_view + i];
+        if (v.width <= 0 || v.height <= 0 || v.stride < v.width || v.alloc_height < v.height) {
+            gray_set_error(GRAY_ERR_INVALID, "render: view %d has bad dimensions", first_view + i);
+            return GRAY_ERR_INVALID;
+        }
+        const bool explicit_rays = v.rays_direction != nullptr;
+        const bool pinhole_possible = !explicit_rays || v.rays_enable != nullptr;
+        if ((explicit_rays && !v.rays_origin) ||
+            (pinhole_possible && (!v.origin || !v.rotation_c2w || !v.vertical_fov)) || !v.znear || !v.zfar ||
+            !v.out_channels || (ctx->n > 0 && !v.rgb)) {
+            gray_set_error(GRAY_ERR_INVALID, "render: view %d lacks a camera, rgb or output pointer", first_view + i);
+            return GRAY_ERR_INVALID;
+        }
+        DevView &d = A.views[i];
+        d.origin = v.origin;
+        d.c2w = v.rotation_c2w;
+        d.fov = v.vertical_fov;
+        d.znear = v.znear;
+        d.zfar = v.zfar;
+        d.rays_o = explicit_rays ? v.rays_origin : nullptr;
+        d.rays_d = explicit_rays ? v.rays_direction : nullptr;
+        d.rays_en = explicit_rays ? v.rays_enable : nullptr;
+        d.rgb = v.rgb;
+        d.out_c = v.out_channels;
+        d.out_depth = v.out_depth;
+        d.out_T = v.out_transmittance;
+        d.out_F = v.out_full_transmittance;
+        d.out_est = v.out_remaining_est;
+        d.out_ro = v.out_ray_origin;
+        d.out_rd = v.out_ray_direction;
+        d.stat_hits = v.stat_hits;
+        d.stat_acc = v.stat_accumulated;
+        d.hit_ids = v.hit_ids;
+        d.hit_T = v.hit_trans;
+        d.n_comp = v.n_comp;
+        d.comp_rgb = v.comp_rgb;
+        d.grad_out = v.grad_out;
+        d.grad_rgb = v.grad_rgb;
+        d.width = v.width;
+        d.height = v.height;
+        d.stride = v.stride;
+        d.alloc_height = v.alloc_height;
+        d.tiles_x = (v.width + GRAY_TILE - 1) / GRAY_TILE;
+        d.tile_base = tiles;
+        d.ray_count = v.stride * v.alloc_height;
+        tiles += d.tiles_x * ((v.height + GRAY_TILE - 1) / GRAY_TILE);
+    }
+    A.num_views = nviews;
+    A.total_tiles = tiles;
+    A.xcd_chunk = (tiles + 7) / 8;
+    A.save = args->save;
+    A.capacity = args->hit_capacity < 0 ? 0 : args->hit_capacity;
+    A.use_bvh = args->use_bvh;
+    A.num_gaussians = (int)ctx->n;
+    A.geo = ctx->geo;
+    A.nodes = ctx->nodes;
+    A.l2w = ctx->l2w;
+    A.error_flag = ctx->err;
+    return GRAY_OK;
+}
+
+int check_render(gray_ctx *ctx, const gray_render_args *args) {
+    if (!ctx || !args || !args->views || args->num_views <= 0) {
+        gray_set_error(GRAY_ERR_INVALID, "render: null context/arguments or no views");
+        return GRAY_ERR_INVALID;
+    }
+    if (!ctx->built) {
+        gray_set_error(GRAY_ERR_STATE, "render: no BVH (call gray_bvh_update with rebuild=1 first)");
+        return GRAY_ERR_STATE;
+    }
+    return GRAY_OK;
+}
+
+}  // namespace
+
+extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, void *stream) {
+    int rc = check_render(ctx, args);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    for (int first = 0; first < args->num_views; first += GRAY_MAX_VIEWS) {
+        const int nv = std::min(GRAY_MAX_VIEWS, args->num_views - first);
+        TraceArgs A;
+        if ((rc = fill_args(ctx, args, first, nv, A))) return rc;
+        if (A.save) {
+            for (int i = 0; i < nv; ++i) {
+                const DevView &d = A.views[i];
+                if (!d.out_T || !d.out_F || !d.out_est || !d.out_ro || !d.out_rd || !d.n_comp || !d.comp_rgb ||
+                    (A.capacity > 0 && (!d.hit_ids || !d.hit_T))) {
+                    gray_set_error(GRAY_ERR_INVALID, "render_forward(save): view %d lacks backward-state buffers",
+                                   first + i);
+                    return GRAY_ERR_INVALID;
+                }
+            }
+        }
+        if (A.total_tiles == 0) continue;
+        hipLaunchKernelGGL(k_trace_forward, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, A.geo, A.nodes,
+                           A.views[0].rgb);
+        GRAY_HIP(hipGetLastError());
+    }
+    return GRAY_OK;
+}
+
+extern "C" int gray_render_backward(gray_ctx *ctx, const gray_render_args *args, const gray_params *params,
+                                    const gray_grads *grads, void *stream) {
+    int rc = check_render(ctx, args);
+    if (rc) return rc;
+    if (!params || !grads || !grads->grad_mean || !grads->grad_rotation || !grads->grad_scale || !grads->grad_opacity) {
+        gray_set_error(GRAY_ERR_INVALID, "render_backward: null params or gradient pointers");
+        return GRAY_ERR_INVALID;
+    }
+    if (params->count != ctx->n) {
+        gray_set_error(GRAY_ERR_STATE, "render_backward: %lld Gaussians but the snapshot has %lld",
+                       (long long)params->count, (long long)ctx->n);
+        return GRAY_ERR_STATE;
+    }
+    if (ctx->n == 0) return GRAY_OK;
+    hipStream_t s = (hipStream_t)stream;
+    for (int first = 0; first < args->num_views; first += GRAY_MAX_VIEWS) {
+        const int nv = std::min(GRAY_MAX_VIEWS, args->num_views - first);
+        TraceArgs A;
+        if ((rc = fill_args(ctx, args, first, nv, A))) return rc;
+        for (int i = 0; i < nv; ++i) {
+            const DevView &d = A.views[i];
+            if (!d.out_T || !d.out_F || !d.out_est || !d.out_ro || !d.out_rd || !d.n_comp || !d.comp_rgb ||
+                !d.grad_out || !d.grad_rgb || (A.capacity > 0 && (!d.hit_ids || !d.hit_T))) {
+                gray_set_error(GRAY_ERR_INVALID, "render_backward: view %d lacks forward state or grad buffers",
+                               first + i);
+                return GRAY_ERR_INVALID;
+            }
+        }
+        A.rotation = params->rotation;
+        A.scale = params->scale;
+        A.opacity = params->opacity;
+        A.g_mean = grads->grad_mean;
+        A.g_rot = grads->grad_rotation;
+        A.g_scale = grads->grad_scale;
+        A.g_op = grads->grad_opacity;
+        A.g_prune = grads->pruning_weight;
+        A.visible = grads->was_visible;
+        if (A.total_tiles == 0) continue;
+        if (A.capacity > 0) {
+            hipLaunchKernelGGL(k_trace_backward_stored, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A);
+            GRAY_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_trace_backward_recompute, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, A.geo,
+                           A.nodes);
+        GRAY_HIP(hipGetLastError());
+    }
+    return GRAY_OK;
+}
+
+extern "C" int gray_debug_hitlists(gray_ctx *ctx, const gray_render_args *args, int32_t max_hits, uint32_t *ids,
+                                   float *dists, int32_t *counts, void *stream) {
+    int rc = check_render(ctx, args);
+    if (rc) return rc;
+    if (!ids || !dists || !counts || max_hits <= 0) {
+        gray_set_error(GRAY_ERR_INVALID, "debug_hitlists: null output or max_hits <= 0");
+        return GRAY_ERR_INVALID;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    TraceArgs A;
+    if ((rc = fill_args(ctx, args, 0, 1, A))) return rc;
+    A.dbg_ids = ids;
+    A.dbg_dist = dists;
+    A.dbg_count = counts;
+    A.dbg_max = max_hits;
+    if (A.total_tiles == 0) return GRAY_OK;
+    hipLaunchKernelGGL(k_debug_hits, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, A.geo, A.nodes);
+    GRAY_HIP(hipGetLastError());
+    const int rays = A.views[0].width * A.views[0].height;
+    hipLaunchKernelGGL(k_debug_sort, dim3(gray_div_up(rays, 64)), dim3(64), 0, s, rays, max_hits, ids, dists, counts);
+    GRAY_HIP(hipGetLastError());
+    return GRAY_OK;
+}

@@ -1,0 +1,259 @@
+/*
+ * gray_hip.h -- C ABI of the MI355X (gfx950) differentiable Gaussian ray tracer.
+ *
+ * Every entry point takes plain device pointers and sizes, a caller-owned HIP
+ * stream (passed as void*, NULL = default stream) and returns an int status
+ * (GRAY_OK or a GRAY_ERR_* code; gray_last_error() gives the message of the
+ * last failure on the calling thread).  Nothing traps on the device: capacity
+ * problems come back as GRAY_ERR_OVERFLOW.  All Gaussian tensors use the
+ * reference layouts (cuda/core/gaussians.h:7-65):
+ *   mean [N,3], rotation [N,4] (r,x,y,z, unnormalised), scale [N,3] (log),
+ *   opacity [N,1] (logit), channels [N,3], sh_coeffs_dc [N,1,3],
+ *   sh_coeffs_rest [N,15,3] (coefficient-major, channel-minor), fp32.
+ * The library owns only the BVH and its build scratch (inside gray_ctx); every
+ * other buffer is allocated by the caller.  One host thread per gray_ctx.
+ *
+ * Each function below names the reference interface it replaces.
+ */
+#ifndef GRAY_HIP_H
+#define GRAY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRAY_ABI_VERSION 1
+
+#define GRAY_OK 0
+#define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
+#define GRAY_ERR_HIP 2      /* a HIP runtime call failed */
+#define GRAY_ERR_OVERFLOW 3 /* a device-side capacity was exceeded (nothing trapped) */
+#define GRAY_ERR_NOMEM 4    /* device allocation failed */
+#define GRAY_ERR_STATE 5    /* call order violated (e.g. render before any BVH build) */
+
+#define GRAY_MAX_ALPHA 0.9999f /* reference CMakeLists.txt:68-71 (MAX_ALPHA) */
+#define GRAY_CHANNELS 3        /* reference CMakeLists.txt:68-71 (CHANNELS) */
+#define GRAY_BATCH 32          /* reference forward_pass.cu:4 (BUFFER_SIZE) */
+#define GRAY_MAX_ROUNDS 99     /* reference forward_pass.cu:3 (MAX_ITERATIONS) */
+
+typedef struct gray_ctx gray_ctx;
+
+/* Runtime knobs, device pointers to 1-element tensors exactly like the
+ * reference ConfigDataHolder (cuda/core/config.h:5-44).  Kernels read them on
+ * the device, so changing a knob with fill_() needs no host round trip. */
+typedef struct {
+    const float *exp_power;                 /* [1] */
+    const float *alpha_threshold;           /* [1] */
+    const float *t_threshold;               /* [1] */
+    const float *global_scale_factor;       /* [1] */
+    const float *eps_forward_normalization; /* [1] */
+    const float *eps_scale_grad;            /* [1] */
+    const float *background_channels;       /* [3] */
+    const uint8_t *render_depth;            /* [1] bool */
+    const uint8_t *enable_sh;               /* [1] bool */
+} gray_knobs;
+
+/* Gaussian parameters (read by every stage) */
+typedef struct {
+    int32_t count;            /* N */
+    int32_t num_sh_coeffs;    /* (max_degree+1)^2-1, 15 for degree 3 */
+    int32_t sh_is_fp16;       /* inference_only layout (gaussians.h:132) */
+    const int32_t *current_sh_degree; /* [1] device */
+    const float *mean;        /* [N,3] */
+    const float *rotation;    /* [N,4] */
+    const float *scale;       /* [N,3] log-scale */
+    const float *opacity;     /* [N,1] logit */
+    const void *sh_dc;        /* [N,1,3] fp32 (or fp16 when sh_is_fp16) */
+    const void *sh_rest;      /* [N,K,3] */
+} gray_params;
+
+/* One camera view of a (possibly multi-view) launch.  Camera values are
+ * device pointers so a caller can update them with fill_/copy_ in place
+ * (reference camera.h:8-38).  Per-pixel buffers are [alloc_height, stride, C]
+ * with pixel_id = y*stride + x (reference shaders.cu:73). */
+typedef struct {
+    const float *origin;        /* [3] camera origin */
+    const float *rotation_c2w;  /* [3,3] row-major camera-to-world (Blender axes) */
+    const float *vertical_fov;  /* [1] radians */
+    const float *znear;         /* [1] */
+    const float *zfar;          /* [1] */
+    const float *rays_origin;   /* [H,stride,3] explicit rays, or NULL for pinhole */
+    const float *rays_direction;/* [H,stride,3] explicit rays, or NULL for pinhole */
+    const uint8_t *rays_enable; /* [1] device flag or NULL; when given, the explicit rays are used only
+                                   while it is nonzero (reference config rays_from_python) */
+    int32_t width, height;      /* active render resolution */
+    int32_t stride;             /* allocated framebuffer width (pixels) */
+    int32_t alloc_height;       /* allocated framebuffer height */
+    /* per-view Gaussian colours [N,3] (SH evaluated for this view) */
+    const float *rgb;
+    /* forward outputs (reference Framebuffer, framebuffer.h:22-53) */
+    float *out_channels;        /* [H,stride,3] always written */
+    float *out_depth;           /* [H,stride,1] or NULL */
+    float *out_transmittance;   /* [H,stride,1] or NULL */
+    float *out_full_transmittance; /* [H,stride,1] or NULL */
+    float *out_remaining_est;   /* [H,stride,3] or NULL */
+    float *out_ray_origin;      /* [H,stride,3] or NULL */
+    float *out_ray_direction;   /* [H,stride,3] or NULL */
+    int32_t *stat_hits;         /* [H,stride] num_gaussians_hit (+=) or NULL */
+    int32_t *stat_accumulated;  /* [H,stride] num_gaussians_accumulated (+=) or NULL */
+    /* backward state written by the forward when save != 0 (caller owned):
+     * hit_ids/hit_trans are [hit_capacity, alloc_height*stride] (slot-major),
+     * n_comp [alloc_height*stride], comp_rgb [alloc_height*stride, 3] */
+    uint32_t *hit_ids;
+    float *hit_trans;
+    uint32_t *n_comp;
+    float *comp_rgb;
+    /* backward inputs / outputs */
+    const float *grad_out;      /* [H,stride,3] dL/d(out_channels) */
+    float *grad_rgb;            /* [N,3] dL/d(rgb) for this view (+=) */
+} gray_view;
+
+typedef struct {
+    gray_knobs knobs;
+    int32_t num_views;
+    const gray_view *views;     /* host array of num_views entries */
+    int32_t save;               /* forward: write backward state + ray outputs */
+    int32_t hit_capacity;       /* L: hits stored per ray (0 = recompute all in backward) */
+    int32_t use_bvh;            /* 1 = BVH traversal, 0 = test every Gaussian (brute force) */
+} gray_render_args;
+
+/* Gradient targets of the backward (reference gaussians.h:34-62). */
+typedef struct {
+    float *grad_mean;           /* [N,3] += */
+    float *grad_rotation;       /* [N,4] += */
+    float *grad_scale;          /* [N,3] += */
+    float *grad_opacity;        /* [N,1] += */
+    float *pruning_weight;      /* [N,1] += (or NULL) */
+    uint8_t *was_visible;       /* [N,1] set to 1 (or NULL) */
+} gray_grads;
+
+/* Adam state (reference cuda/opt/adam.cu:6-84, gaussians.h:21-56). */
+typedef struct {
+    float *param[7];   /* mean, rotation, scale, opacity, channels, sh_dc, sh_rest */
+    float *grad[7];
+    float *m1[7];
+    float *m2[7];
+    const float *lr[7];      /* device [1] (first element read, adam.cu:33) */
+    int64_t numel[7];        /* floats per tensor */
+    const float *beta1;      /* device [1] */
+    const float *beta2;      /* device [1] */
+    const float *epsilon;    /* device [1] */
+    /* pruning statistics fused into the step (gray/raytracer.py:361-366) */
+    const uint8_t *was_visible; /* [N] or NULL */
+    int32_t *pruning_counter;   /* [N] or NULL */
+    int32_t count;              /* N */
+    /* optional device-resident switches (reference ConfigDataHolder and
+     * GaussianDataHolder fields); when set they override the host flags, so
+     * a step needs no device->host read */
+    const uint8_t *update_channels;    /* [1] bool or NULL */
+    const uint8_t *zero_grads;         /* [1] bool or NULL */
+    const uint8_t *enable_sh;          /* [1] bool or NULL */
+    const int32_t *sh_update_laziness; /* [1] or NULL */
+} gray_adam;
+
+/* ---- context -------------------------------------------------------------- */
+
+/* Replaces the torch custom-class ctor gray.Raytracer (cuda/raytracer.h:49-86).
+ * Creates the BVH holder on the current HIP device. */
+int gray_ctx_create(gray_ctx **out);
+void gray_ctx_destroy(gray_ctx *ctx);
+const char *gray_last_error(void);
+int gray_abi_version(void);
+
+/* No reference counterpart (the reference traps on the device,
+ * per_pixel_linked_list.h:23-26).  Synchronises the stream, then reports and
+ * clears the device-side error flag (e.g. GRAY_ERR_OVERFLOW for a traversal
+ * stack overflow) of every launch since the previous call. */
+int gray_device_status(gray_ctx *ctx, void *stream);
+
+/* ---- geometry + BVH ------------------------------------------------------- */
+
+/* Replaces BVHWrapper::rebuild / BVHWrapper::update and populate_bvh_kernel
+ * (cuda/optix/bvh_wrapper.h:15-39, bvh_wrapper.cu:8-53, raytracer.h:135-142).
+ * Snapshots the per-Gaussian geometry (opacity, clip factor, world-to-local
+ * transform, exact ellipsoid AABB) and either rebuilds the LBVH (rebuild != 0:
+ * Morton codes, LDS radix sort, Karras topology, refit) or refits the existing
+ * topology.  A refit needs an earlier rebuild with the same N. */
+int gray_bvh_update(gray_ctx *ctx, const gray_params *params, const gray_knobs *knobs, int rebuild,
+                    void *stream);
+
+/* Milliseconds of the last rebuild (device time, measured with HIP events),
+ * or -1 if none was timed.  Timing is only recorded when enabled. */
+int gray_bvh_set_timing(gray_ctx *ctx, int enable);
+float gray_bvh_last_build_ms(gray_ctx *ctx);
+
+/* Number of Gaussians and internal nodes of the current BVH. */
+int gray_bvh_info(gray_ctx *ctx, int64_t *num_gaussians, int64_t *num_nodes, int64_t *depth_bound);
+
+/* ---- spherical harmonics -------------------------------------------------- */
+
+/* Replaces sh_forward_pass / sh_forward_kernel (cuda/utils/sh.cu:224-247,
+ * 291-298): rgb_out[v] = max(C0*dc + sum Y_k(dir)*c_k + 0.5, 0) with
+ * dir = normalize(mean - origin[v]).  origins: host array of num_views device
+ * pointers to [3]; rgb_out: host array of num_views device pointers to [N,3].
+ * Returns immediately (no work) when *knobs->enable_sh is 0 on the device. */
+int gray_sh_forward(gray_ctx *ctx, const gray_params *params, const gray_knobs *knobs, int32_t num_views,
+                    const float *const *origins, float *const *rgb_out, void *stream);
+
+/* Replaces sh_backward_pass / sh_backward_kernel (sh.cu:249-287, 300-304).
+ * For Gaussians with was_visible: masks grad_rgb where rgb <= 0, writes
+ * dL/d(sh_dc), dL/d(sh_rest) and adds the view-direction term to grad_mean.
+ * accumulate == 0 reproduces the reference assignment semantics for the
+ * dc/rest gradients (single view); accumulate != 0 sums over the views. */
+int gray_sh_backward(gray_ctx *ctx, const gray_params *params, const gray_knobs *knobs, int32_t num_views,
+                     const float *const *origins, const float *const *rgb, const float *const *grad_rgb,
+                     const uint8_t *was_visible, float *grad_sh_dc, float *grad_sh_rest, float *grad_mean,
+                     int accumulate, void *stream);
+
+/* ---- render --------------------------------------------------------------- */
+
+/* Replaces the forward half of Raytracer::forward_pass (raytracer.h:88-102):
+ * __raygen__rg + __intersection__is + forward_pass (shaders.cu:10-106,
+ * forward_pass.cu:6-151).  Needs a prior gray_bvh_update with the same N.
+ * Wavefront-coherent BVH traversal (one 8x8 ray tile per wave64), analytic
+ * ray-ellipsoid test, per-ray sorted 32-entry k-buffer in LDS, front-to-back
+ * compositing in rounds of 32 with the reference truncation estimate. */
+int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, void *stream);
+
+/* Replaces Raytracer::backward_pass (raytracer.h:116-124, backward_pass.cu:2-142):
+ * replays each ray's composited hits (stored by the forward, or recomputed by
+ * re-traversal for rays whose list exceeded hit_capacity) back to front and
+ * scatters dL/d(mean, rotation, scale, opacity) into grads and dL/d(rgb) into
+ * view.grad_rgb.  Uses the geometry snapshot of the forward. */
+int gray_render_backward(gray_ctx *ctx, const gray_render_args *args, const gray_params *params,
+                         const gray_grads *grads, void *stream);
+
+/* ---- optimiser ------------------------------------------------------------ */
+
+/* Replaces adam_step + adam_step_base_kernel + adam_step_sh_kernel
+ * (cuda/opt/adam.cu:25-84, raytracer.h:126-133) and the pruning counter update
+ * of Raytracer.update_pruning_stats (gray/raytracer.py:361-366), in ONE
+ * launch.  step is total_num_steps after the increment; the bias correction
+ * uses step+1 like the reference (adam.cu:16-17).  flags: bit0 update_channels,
+ * bit1 zero_grads, bit2 update SH (enable_sh && step % sh_update_laziness == 0);
+ * each is replaced by the device-resident switch in gray_adam when given. */
+int gray_adam_step(gray_ctx *ctx, const gray_adam *adam, int32_t step, int32_t flags, void *stream);
+
+/* ---- debugging / parity --------------------------------------------------- */
+
+/* No reference counterpart (the reference keeps hits in its per-pixel linked
+ * lists, per_pixel_linked_list.h:5-64).  For every ray of view 0 writes the
+ * sorted (distance, Gaussian id) list of ALL hits passing the hit criterion
+ * of shaders.cu:22-47, capped at max_hits: ids/dists are [R, max_hits],
+ * counts [R] (true count, may exceed max_hits).  Used for bit-exact hit-set
+ * checks against the CPU oracle and between BVH and brute-force traversal. */
+int gray_debug_hitlists(gray_ctx *ctx, const gray_render_args *args, int32_t max_hits, uint32_t *ids,
+                        float *dists, int32_t *counts, void *stream);
+
+/* Copies the geometry snapshot of the last gray_bvh_update to caller buffers
+ * (device): geo [N,16] = {W2L 3x3 row-major, mean 3, opacity, clip, 2 pad},
+ * aabb [N,6] = {min xyz, max xyz}.  Either pointer may be NULL. */
+int gray_debug_snapshot(gray_ctx *ctx, float *geo, float *aabb, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAY_HIP_H */

@@ -1,0 +1,71 @@
+"""Shared set-up for the GPU parity tests (test infrastructure)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from gray_amd.camera import CameraInfo
+from gray_amd.native import NativeRaytracer
+
+
+def setup_native(scene, cam: CameraInfo, alpha_threshold=0.01, t_threshold=0.03, exp_power=2.0, bg=(0, 0, 0),
+                 sh=True, sh_degree=3, channels=None, hit_capacity=None, znear=0.0, zfar=99999.9):
+    n = scene.count if scene is not None else 0
+    W, H = cam.image_width, cam.image_height
+    nr = NativeRaytracer(W, H, n, 3, 300_000_000, 120_000_000, hit_capacity=hit_capacity)
+    g = nr.get_gaussians()
+    cfg = nr.get_config()
+    cfg.alpha_threshold.fill_(alpha_threshold)
+    cfg.t_threshold.fill_(t_threshold)
+    cfg.exp_power.fill_(exp_power)
+    cfg.background_channels.copy_(torch.tensor(bg, dtype=torch.float32))
+    cfg.enable_sh.fill_(sh)
+    cfg.update_channels.fill_(not sh)
+    if n > 0:
+        with torch.no_grad():
+            g.mean.copy_(scene.mean)
+            g.rotation.copy_(scene.rotation)
+            g.scale.copy_(scene.scale)
+            g.opacity.copy_(scene.opacity)
+            g.sh_coeffs_dc.copy_(scene.sh_dc)
+            g.sh_coeffs_rest.copy_(scene.sh_rest)
+            if channels is not None:
+                g.channels.copy_(channels)
+            g.current_sh_degree.fill_(sh_degree)
+    set_camera(nr, cam, znear, zfar)
+    nr.rebuild_bvh()
+    return nr
+
+
+def set_camera(nr: NativeRaytracer, cam: CameraInfo, znear=0.0, zfar=99999.9):
+    c = nr.get_camera()
+    c.znear.fill_(znear)
+    c.zfar.fill_(zfar)
+    c.vertical_fov_radians.fill_(float(cam.fov_y))
+    c.set_pose(cam.origin_cuda(), cam.rotation_c2w_blender_cuda())
+
+
+def mock_camera(n=32) -> CameraInfo:
+    """The reference tests' camera: origin (0,0,1), R = -I, fov_y 1.5."""
+    return CameraInfo(uid=0, R=-np.eye(3), T=np.zeros(3), origin=np.array([0.0, 0.0, 1.0]), fov_y=1.5,
+                      fov_x=1.5, image_width=n, image_height=n)
+
+
+def rays_of(nr: NativeRaytracer):
+    fb = nr.get_framebuffer()
+    h, w = nr.height, nr.width
+    ro = fb.ray_origin[:h, :w].reshape(-1, 3).cpu().numpy()
+    rd = fb.ray_direction[:h, :w].reshape(-1, 3).cpu().numpy()
+    return ro, rd
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64).reshape(-1)
+    b = np.asarray(b, np.float64).reshape(-1)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-12)
+
+
+def deg(x):
+    return math.radians(x)

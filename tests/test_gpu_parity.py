@@ -1,0 +1,295 @@
+"""GPU (HIP) path against the CPU oracle on the same seeded inputs.
+
+Bars: hit sets/order/distances bit-exact (same snapshot and rays); image
+within 1e-4 max-abs (1e-5 with the GPU's own snapshot and rays); gradients
+within 1e-3 relative to the largest component."""
+import numpy as np
+import pytest
+import torch
+
+from gray_amd import scenes
+from oracle import oracle as O
+
+from gpu_util import mock_camera, rays_of, rel_err, setup_native
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tiny():
+    torch.manual_seed(0)
+    scene = scenes.tiny_scene(1024, seed=0)
+    cam = scenes.tiny_camera(64, 64)
+    nr = setup_native(scene, cam)
+    torch.set_grad_enabled(True)
+    nr.forward_pass()
+    torch.cuda.synchronize()
+    nr.device_status()
+    return scene, cam, nr
+
+
+def oracle_knobs(nr):
+    c = nr.get_config()
+    cam = nr.get_camera()
+    return O.make_knobs(exp_power=float(c.exp_power), alpha_threshold=float(c.alpha_threshold),
+                        t_threshold=float(c.t_threshold), eps_fwd=float(c.eps_forward_normalization),
+                        eps_scale_grad=float(c.eps_scale_grad), bg=tuple(c.background_channels.tolist()),
+                        znear=float(cam.znear), zfar=float(cam.zfar), render_depth=bool(c.render_depth))
+
+
+def test_snapshot_matches_oracle(tiny):
+    scene, cam, nr = tiny
+    geo, aabb = nr.debug_snapshot()
+    geo = geo.cpu().numpy()
+    ogeo, _ = O.snapshot(scene.mean, scene.rotation, scene.scale, scene.opacity, 0.01, 2.0)
+    vis = ogeo[:, 12] > 0
+    assert np.array_equal(vis, geo[:, 12] > 0)
+    assert np.abs(geo[:, 9:12] - ogeo[:, 9:12]).max() == 0.0
+    for row in range(3):
+        a, b = geo[vis, 3 * row:3 * row + 3], ogeo[vis, 3 * row:3 * row + 3]
+        scale = np.abs(b).max(axis=1, keepdims=True)
+        assert np.all(np.abs(a - b) <= 1e-5 * scale), row
+    for col in (12, 13):
+        a, b = geo[vis, col], ogeo[vis, col]
+        assert np.all(np.abs(a - b) <= 4e-6 * np.abs(b)), col
+    # AABB contains every ellipsoid point (sample the surface)
+    box = aabb.cpu().numpy()
+    l2w = np.linalg.inv(ogeo[:, :9].reshape(-1, 3, 3).astype(np.float64))
+    u = np.random.default_rng(0).normal(size=(64, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    pts = ogeo[:, None, 9:12] + np.einsum("nij,kj->nki", l2w, u)
+    assert np.all(pts[vis] >= box[vis, None, :3] - 1e-6) and np.all(pts[vis] <= box[vis, None, 3:] + 1e-6)
+
+
+def test_rays_match_oracle(tiny):
+    scene, cam, nr = tiny
+    ro, rd = rays_of(nr)
+    oro, ord_ = O.pinhole_rays(64, 64, cam.fov_y, cam.origin.astype(np.float32), cam.rotation_c2w_blender())
+    assert np.abs(ro - oro).max() == 0.0
+    assert np.abs(rd - ord_).max() < 2e-6
+
+
+def test_sh_colours_match_oracle(tiny):
+    scene, cam, nr = tiny
+    rgb = nr.get_gaussians().channels.cpu().numpy()
+    orgb = O.sh_forward(scene.mean, scene.sh_dc, scene.sh_rest, cam.origin.astype(np.float32), 3)
+    assert np.abs(rgb - orgb).max() < 2e-6
+
+
+@pytest.mark.parametrize("use_bvh", [True, False])
+def test_hit_lists_bit_exact(tiny, use_bvh):
+    scene, cam, nr = tiny
+    geo, _ = nr.debug_snapshot()
+    geo = geo.cpu().numpy()
+    ro, rd = rays_of(nr)
+    ids, dists, counts = nr.debug_hitlists(256, use_bvh=use_bvh)
+    ids, dists, counts = ids.cpu().numpy(), dists.cpu().numpy(), counts.cpu().numpy()
+    oids, odists, ocounts = O.hitlists(geo, ro, rd, 0.01, 256)
+    assert counts.max() < 256
+    assert np.array_equal(counts, ocounts)
+    assert counts.sum() > 1000, "scene must produce hits"
+    for r in range(len(counts)):
+        c = counts[r]
+        assert np.array_equal(ids[r, :c].astype(np.uint32), oids[r, :c])
+        assert np.array_equal(dists[r, :c].view(np.uint32), odists[r, :c].view(np.uint32))
+
+
+def test_forward_matches_oracle(tiny):
+    scene, cam, nr = tiny
+    fb = nr.get_framebuffer()
+    img = fb.output_channels[:64, :64].reshape(-1, 3).cpu().numpy()
+    geo, _ = nr.debug_snapshot()
+    ro, rd = rays_of(nr)
+    rgb = nr.get_gaussians().channels.cpu().numpy()
+    k = oracle_knobs(nr)
+    same = O.render(geo.cpu().numpy(), rgb, ro, rd, k)
+    assert np.abs(img - same["C"]).max() < 1e-5
+    T = fb.transmittance[:64, :64].reshape(-1).cpu().numpy()
+    F = fb.full_transmittance[:64, :64].reshape(-1).cpu().numpy()
+    assert np.abs(T - same["T"]).max() < 1e-5 and np.abs(F - same["F"]).max() < 1e-5
+    ncomp = nr.hits.n_comp[: 64 * 64].cpu().numpy()
+    assert (ncomp != same["ncomp"]).sum() <= 2
+    # stored composited lists replay the oracle's front-to-back order
+    cap = nr.hits.capacity
+    hid = nr.hits.hit_ids[:, : 64 * 64].cpu().numpy().T
+    for r in np.nonzero(ncomp == same["ncomp"])[0]:
+        c = min(ncomp[r], cap)
+        assert np.array_equal(hid[r, :c].astype(np.uint32), same["ids"][r, :c])
+    # full oracle: its own snapshot, rays and SH
+    ogeo, _ = O.snapshot(scene.mean, scene.rotation, scene.scale, scene.opacity, 0.01, 2.0)
+    oro, ord_ = O.pinhole_rays(64, 64, cam.fov_y, cam.origin.astype(np.float32), cam.rotation_c2w_blender())
+    orgb = O.sh_forward(scene.mean, scene.sh_dc, scene.sh_rest, cam.origin.astype(np.float32), 3)
+    full = O.render(ogeo, orgb, oro, ord_, k)
+    assert np.abs(img - full["C"]).max() < 1e-4
+
+
+def test_backward_matches_oracle(tiny):
+    scene, cam, nr = tiny
+    fb = nr.get_framebuffer()
+    g = nr.get_gaussians()
+    img = fb.output_channels[:64, :64].clone().requires_grad_()
+    target = torch.rand(64, 64, 3, generator=torch.Generator().manual_seed(1)).cuda()
+    loss = torch.nn.functional.l1_loss(img, target)
+    loss.backward()
+    with torch.no_grad():
+        fb.grad_output_channels[:64, :64].copy_(img.grad)
+        for name in ("grad_mean", "grad_rotation", "grad_scale", "grad_opacity", "grad_channels",
+                     "grad_sh_coeffs_dc", "grad_sh_coeffs_rest", "pruning_weight"):
+            getattr(g, name).zero_()
+    nr.backward_pass()
+    torch.cuda.synchronize()
+    nr.device_status()
+    geo, _ = nr.debug_snapshot()
+    geo = geo.cpu().numpy()
+    _, l2w = O.snapshot(scene.mean, scene.rotation, scene.scale, scene.opacity, 0.01, 2.0)
+    ro, rd = rays_of(nr)
+    rgb = g.channels.cpu().numpy()
+    k = oracle_knobs(nr)
+    fwd = O.render(geo, rgb, ro, rd, k)
+    gout = img.grad.reshape(-1, 3).cpu().numpy()
+    ob = O.backward(geo, l2w, rgb, scene.rotation, scene.scale, scene.opacity, ro, rd, fwd, gout, k, 64 * 64)
+    vis = g.was_visible.reshape(-1).cpu().numpy().astype(np.uint8)
+    gdc, grest, gmean = O.sh_backward(scene.mean, scene.sh_rest, cam.origin.astype(np.float32), rgb, ob["rgb"],
+                                      vis, 3, grad_mean=ob["mean"])
+    assert rel_err(g.grad_channels.cpu(), ob["rgb"]) < 1e-3
+    assert rel_err(g.grad_opacity.cpu(), ob["opacity"]) < 1e-3
+    assert rel_err(g.grad_scale.cpu(), ob["scale"]) < 1e-3
+    assert rel_err(g.grad_rotation.cpu(), ob["rotation"]) < 1e-3
+    assert rel_err(g.grad_mean.cpu(), gmean) < 1e-3
+    assert rel_err(g.grad_sh_coeffs_dc.cpu(), gdc) < 1e-3
+    assert rel_err(g.grad_sh_coeffs_rest.cpu(), grest) < 1e-3
+    assert rel_err(g.pruning_weight.cpu(), ob["pruning"]) < 1e-3
+    assert np.array_equal(vis > 0, (np.abs(ob["opacity"].reshape(-1)) > 0) | (ob["pruning"].reshape(-1) > 0))
+
+
+def test_recompute_backward_equals_stored():
+    scene = scenes.tiny_scene(1024, seed=3)
+    cam = scenes.tiny_camera(48, 40)
+    grads = []
+    for cap in (64, 0, 3):
+        nr = setup_native(scene, cam, t_threshold=0.001, hit_capacity=cap)
+        torch.set_grad_enabled(True)
+        nr.forward_pass()
+        fb = nr.get_framebuffer()
+        fb.grad_output_channels.copy_(torch.randn(fb.grad_output_channels.shape,
+                                                  generator=torch.Generator().manual_seed(5)).cuda())
+        g = nr.get_gaussians()
+        for name in ("grad_mean", "grad_rotation", "grad_scale", "grad_opacity", "grad_channels"):
+            getattr(g, name).zero_()
+        nr.backward_pass()
+        torch.cuda.synchronize()
+        nr.device_status()
+        grads.append({k: getattr(g, k).cpu().numpy().copy() for k in
+                      ("grad_mean", "grad_rotation", "grad_scale", "grad_opacity", "grad_channels")})
+    for other in grads[1:]:
+        for key in grads[0]:
+            assert rel_err(other[key], grads[0][key]) < 2e-4, key
+
+
+@pytest.mark.parametrize("athr", [0.01, 0.3])
+def test_reference_single_gaussian(athr):
+    from gray_amd import Raytracer, RaytracerConfig
+    from oracle import torch_ref
+    from test_oracle import target_image
+
+    cfg = RaytracerConfig(sh=False)
+    rt = Raytracer(cfg, 1, 32, 32)
+    config = rt.cuda_module.get_config()
+    config.zero_grads.fill_(False)
+    config.alpha_threshold.fill_(athr)
+    config.background_channels.copy_(torch.tensor([0.1, 0.1, 0.1]))
+    mu = np.array([[-0.05, -0.05, 0.0]])
+    ls = np.log([[0.1, 0.3, 0.3]])
+    col = np.array([[1.0, 0.0, 0.0]])
+    q = np.array([[-0.5, 0.0, 0.0, 1.0]])
+    lo = np.array([[0.0]])
+    g = rt.cuda_module.get_gaussians()
+    with torch.no_grad():
+        for name, v in (("mean", mu), ("scale", ls), ("channels", col), ("rotation", q), ("opacity", lo)):
+            getattr(g, name).copy_(torch.tensor(v, dtype=torch.float32))
+    rt.cuda_module.rebuild_bvh()
+    render = rt(mock_camera(32))
+    target = torch.tensor(target_image()).permute(2, 0, 1).cuda()
+    loss = torch.nn.functional.l1_loss(render, target)
+    rt.backward(loss)
+    fb = rt.cuda_module.get_framebuffer()
+    ro = fb.ray_origin.reshape(-1, 3).cpu().double()
+    rd = fb.ray_direction.reshape(-1, 3).cpu().double()
+    t = lambda a: torch.tensor(np.asarray(a, np.float64), requires_grad=True)  # noqa: E731
+    tm, ts, tc, tq, to = t(mu), t(ls), t(col), t(q), t(lo)
+    ref, _ = torch_ref.render(tm, tq, ts, to, tc, ro, rd, 2.0, athr, 0.03, (0.1, 0.1, 0.1))
+    ref_img = ref.reshape(32, 32, 3).permute(2, 0, 1)
+    diff = (render.detach().cpu().double() - ref_img.detach()).abs()
+    assert diff.mean() < 1e-5 and diff.max() < 1e-4
+    torch.nn.functional.l1_loss(ref_img, target.cpu().double()).backward()
+    for name, ref_t in (("mean", tm), ("rotation", tq), ("scale", ts), ("opacity", to), ("channels", tc)):
+        assert (getattr(g, name).grad.cpu().double() - ref_t.grad).abs().max() < 1e-4, name
+
+
+@pytest.mark.parametrize("tthr", [0.0, 0.5])
+def test_reference_multiple_gaussians(tthr):
+    from gray_amd import Raytracer, RaytracerConfig
+    from oracle import torch_ref
+    from test_oracle import target_image
+
+    cfg = RaytracerConfig(t_threshold=tthr, sh=False)
+    rt = Raytracer(cfg, 3, 32, 32)
+    rt.cuda_module.get_config().zero_grads.fill_(False)
+    mu = np.array([[0.05, 0.05, -0.25], [-0.05, -0.05, 0.25], [0.0, 0.0, 0.0]])
+    ls = np.log([[1.0, 0.2, 0.1], [0.1, 0.3, 0.3], [0.4, 0.4, 0.4]])
+    col = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    q = np.array([[1.0, 0.0, 0.0, 1.0]] * 3)
+    opv = np.array([[0.5], [0.7], [0.7]])
+    lo = np.log(opv / (1 - opv))
+    g = rt.cuda_module.get_gaussians()
+    with torch.no_grad():
+        for name, v in (("mean", mu), ("scale", ls), ("channels", col), ("rotation", q), ("opacity", lo)):
+            getattr(g, name).copy_(torch.tensor(v, dtype=torch.float32))
+    rt.cuda_module.rebuild_bvh()
+    render = rt(mock_camera(32))
+    target = torch.tensor(target_image()).permute(2, 0, 1).cuda()
+    rt.backward(torch.nn.functional.l1_loss(render, target))
+    fb = rt.cuda_module.get_framebuffer()
+    ro = fb.ray_origin.reshape(-1, 3).cpu().double()
+    rd = fb.ray_direction.reshape(-1, 3).cpu().double()
+    t = lambda a: torch.tensor(np.asarray(a, np.float64), requires_grad=True)  # noqa: E731
+    tm, ts, tc, tq, to = t(mu), t(ls), t(col), t(q), t(lo)
+    ref, _ = torch_ref.render(tm, tq, ts, to, tc, ro, rd, 2.0, 0.01, tthr, (0, 0, 0))
+    ref_img = ref.reshape(32, 32, 3).permute(2, 0, 1)
+    diff = (render.detach().cpu().double() - ref_img.detach()).abs()
+    assert diff.mean() < 1e-5 and diff.max() < 1e-4
+    torch.nn.functional.l1_loss(ref_img, target.cpu().double()).backward()
+    for name, ref_t in (("mean", tm), ("rotation", tq), ("scale", ts), ("opacity", to), ("channels", tc)):
+        assert (getattr(g, name).grad.cpu().double() - ref_t.grad).abs().max() < 1e-4, name
+
+
+def test_adam_matches_oracle():
+    scene = scenes.tiny_scene(700, seed=9)
+    cam = scenes.tiny_camera(16, 16)
+    nr = setup_native(scene, cam)
+    g = nr.get_gaussians()
+    gen = torch.Generator().manual_seed(2)
+    host = {}
+    for name in g.PARAMS:
+        grad = torch.randn(getattr(g, name).shape, generator=gen)
+        getattr(g, "grad_" + name).copy_(grad)
+        host[name] = [getattr(g, name).cpu().numpy().copy(), grad.numpy().copy(),
+                      np.zeros(grad.shape, np.float32), np.zeros(grad.shape, np.float32)]
+    lrs = dict(zip(g.PARAMS, (1.6e-4, 4e-3, 5e-3, 2e-2, 2.5e-3, 2.5e-3, 6e-4)))
+    for name, lr_name in zip(g.PARAMS, g.LRS):
+        getattr(g, lr_name).fill_(lrs[name])
+    g.was_visible.fill_(True)
+    nr.step(update_pruning_stats=True)
+    torch.cuda.synchronize()
+    for name in g.PARAMS:
+        p, gr, m, v = host[name]
+        if name == "channels":
+            # SH on: channels are not optimised, their gradient is zeroed (adam.cu:35-40)
+            assert np.all(getattr(g, "grad_" + name).cpu().numpy() == 0)
+            continue
+        O.adam(p, gr, m, v, lrs[name], 0.9, 0.999, 1e-15, 1)
+        assert np.abs(getattr(g, name).cpu().numpy() - p).max() < 1e-6, name
+        assert np.abs(getattr(g, "first_moment_" + name).cpu().numpy() - m).max() < 1e-6
+        assert np.abs(getattr(g, "second_moment_" + name).cpu().numpy() - v).max() < 1e-6
+        assert np.all(getattr(g, "grad_" + name).cpu().numpy() == 0)
+    assert np.all(g.pruning_counter.cpu().numpy() == 1)

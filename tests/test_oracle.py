@@ -1,0 +1,386 @@
+"""Pins the CPU oracle (oracle/gray_oracle.c) before it is trusted as the GPU
+checker: closed forms, invariants, the reference tests' known-answer scenes
+(tests/test_single_gaussian.py, tests/test_multiple_gaussians.py of the
+reference, restated), autograd of an independent float64 formulation, and
+finite differences.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from oracle import torch_ref
+
+MAX_ALPHA = 0.9999
+
+
+# ---------------------------------------------------------------------------
+# spherical harmonics: reference complex recursion vs closed form
+# ---------------------------------------------------------------------------
+def closed_form_basis(x, y, z):
+    pi = math.pi
+    c1 = math.sqrt(3 / (4 * pi))
+    c2a, c2b, c2c = math.sqrt(15 / (4 * pi)), math.sqrt(5 / (16 * pi)), math.sqrt(15 / (16 * pi))
+    c3a, c3b, c3c = math.sqrt(35 / (32 * pi)), math.sqrt(105 / (4 * pi)), math.sqrt(21 / (32 * pi))
+    c3d, c3e = math.sqrt(7 / (16 * pi)), math.sqrt(105 / (16 * pi))
+    xx, yy, zz = x * x, y * y, z * z
+    return [
+        -c1 * y, c1 * z, -c1 * x,
+        c2a * x * y, -c2a * y * z, c2b * (2 * zz - xx - yy), -c2a * x * z, c2c * (xx - yy),
+        -c3a * y * (3 * xx - yy), c3b * x * y * z, -c3c * y * (4 * zz - xx - yy),
+        c3d * z * (2 * zz - 3 * xx - 3 * yy), -c3c * x * (4 * zz - xx - yy), c3e * z * (xx - yy),
+        -c3a * x * (xx - 3 * yy),
+    ]
+
+
+def test_closed_form_is_orthonormal():
+    # Gauss-Legendre x uniform-phi quadrature is exact for these polynomials
+    ct, wt = np.polynomial.legendre.leggauss(16)
+    phi = np.linspace(0, 2 * np.pi, 32, endpoint=False)
+    Y = []
+    W = []
+    for c, w in zip(ct, wt):
+        s = math.sqrt(1 - c * c)
+        for p in phi:
+            x, y, z = s * math.cos(p), s * math.sin(p), c
+            Y.append(np.array([1 / (2 * math.sqrt(math.pi))] + closed_form_basis(x, y, z)))
+            W.append(w * 2 * np.pi / len(phi))
+    Y = np.array(Y)
+    W = np.array(W)
+    gram = (Y * W[:, None]).T @ Y
+    assert np.abs(gram - np.eye(16)).max() < 1e-12
+
+
+@pytest.mark.parametrize("degree", [1, 2, 3])
+def test_sh_recursion_matches_closed_form(oracle_lib, degree):
+    rng = np.random.default_rng(degree)
+    kc = (degree + 1) ** 2 - 1
+    for _ in range(200):
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        b, _, _, _ = O.sh_basis(degree, *v)
+        ref = np.array(closed_form_basis(*v))[:kc]
+        assert np.abs(b[:kc] - ref).max() < 2e-6
+
+
+def test_sh_recursion_partials_tangential(oracle_lib):
+    # partials of the recursion agree with the closed form along the sphere
+    rng = np.random.default_rng(7)
+    for _ in range(100):
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        _, bx, by, bz = O.sh_basis(3, *v)
+        t = np.cross(v, rng.normal(size=3))
+        t /= np.linalg.norm(t)
+        h = 1e-5
+        fd = (np.array(closed_form_basis(*(v + h * t))) - np.array(closed_form_basis(*(v - h * t)))) / (2 * h)
+        an = bx * t[0] + by * t[1] + bz * t[2]
+        assert np.abs(an - fd).max() < 5e-5
+
+
+def test_sh_forward_backward_against_autograd(oracle_lib):
+    rng = np.random.default_rng(3)
+    n = 64
+    mean = rng.normal(size=(n, 3)).astype(np.float32)
+    dc = rng.normal(0.5, 0.2, size=(n, 1, 3)).astype(np.float32)
+    rest = rng.normal(0, 0.3, size=(n, 15, 3)).astype(np.float32)
+    origin = np.array([0.3, -2.0, 4.0], np.float32)
+    rgb = O.sh_forward(mean, dc, rest, origin, 3)
+    # float64 autograd of the closed form
+    m = torch.tensor(mean, dtype=torch.float64, requires_grad=True)
+    d = torch.tensor(dc, dtype=torch.float64, requires_grad=True)
+    r = torch.tensor(rest, dtype=torch.float64, requires_grad=True)
+    delta = m - torch.tensor(origin, dtype=torch.float64)
+    dirs = delta / delta.norm(dim=1, keepdim=True)
+    x, y, z = dirs[:, 0], dirs[:, 1], dirs[:, 2]
+    B = torch.stack([torch.as_tensor(v) for v in closed_form_basis(x, y, z)], 1)
+    out = (0.28209479177387814 * d[:, 0] + (B[:, :, None] * r).sum(1) + 0.5).clamp(min=0)
+    assert np.abs(out.detach().numpy() - rgb).max() < 1e-5
+    g = torch.tensor(rng.normal(size=(n, 3)))
+    (out * g).sum().backward()
+    gdc, grest, gmean = O.sh_backward(mean, rest, origin, rgb, g.numpy().astype(np.float32), None, 3)
+    assert np.abs(gdc - d.grad.numpy()).max() < 1e-5
+    assert np.abs(grest - r.grad.numpy()).max() < 1e-5
+    assert np.abs(gmean - m.grad.numpy()).max() < 1e-4 * max(1.0, np.abs(m.grad.numpy()).max())
+
+
+# ---------------------------------------------------------------------------
+# Adam: reference bias correction uses step+1 (adam.cu:16-17)
+# ---------------------------------------------------------------------------
+def test_adam_matches_torch_with_step_offset(oracle_lib):
+    rng = np.random.default_rng(0)
+    p0 = rng.normal(size=257).astype(np.float32)
+    p = p0.copy()
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    tp = torch.tensor(p0.astype(np.float64), requires_grad=True)
+    opt = torch.optim.Adam([tp], lr=1e-2, betas=(0.9, 0.999), eps=1e-15)
+    opt.state[tp] = {"step": torch.tensor(1.0), "exp_avg": torch.zeros_like(tp), "exp_avg_sq": torch.zeros_like(tp)}
+    for step in range(1, 6):
+        g = rng.normal(size=p.shape).astype(np.float32)
+        gg = g.copy()
+        O.adam(p, gg, m, v, 1e-2, 0.9, 0.999, 1e-15, step, zero_grads=True)
+        assert np.all(gg == 0)
+        tp.grad = torch.tensor(g.astype(np.float64))
+        opt.step()
+        assert np.abs(p - tp.detach().numpy()).max() < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# camera and snapshot closed forms
+# ---------------------------------------------------------------------------
+def test_pinhole_rays_closed_form(oracle_lib):
+    W, H, fov = 64, 48, 1.1
+    c2w = np.diag([-1.0, 1.0, 1.0]).astype(np.float32)
+    ro, rd = O.pinhole_rays(W, H, fov, np.array([0, 0, 1], np.float32), c2w)
+    assert np.allclose(np.linalg.norm(rd, axis=1), 1.0, atol=1e-6)
+    v = math.tan(fov / 2)
+    a = W / H
+    for (i, j) in [(0, 0), (W - 1, 0), (W // 2, H // 2), (5, 17)]:
+        x = a * v * (2 * (i + 0.5) / W - 1)
+        y = v * (1 - 2 * (j + 0.5) / H)
+        d = c2w[:, 0] * x + c2w[:, 1] * y - c2w[:, 2]
+        d /= np.linalg.norm(d)
+        assert np.abs(rd[j * W + i] - d).max() < 1e-6
+
+
+def test_snapshot_clip_invariant(oracle_lib):
+    # at the clip boundary |x| = 1 the kernel times opacity equals alpha_threshold
+    rng = np.random.default_rng(1)
+    n = 50
+    for p in (1.0, 2.0, 3.0, 2.5):
+        logit = rng.normal(size=(n, 1)).astype(np.float32) + 1.0
+        geo, _ = O.snapshot(np.zeros((n, 3)), np.tile([1, 0, 0, 0], (n, 1)), np.zeros((n, 3)), logit,
+                            alpha_threshold=0.01, exp_power=p)
+        op, clip = geo[:, 12], geo[:, 13]
+        vis = op > 0
+        g = np.exp(-(clip[vis].astype(np.float64) ** 2) ** p / (2 * p))
+        assert np.abs(op[vis] * g - 0.01).max() < 1e-6
+
+
+def test_snapshot_world_to_local(oracle_lib):
+    rng = np.random.default_rng(2)
+    n = 40
+    mean = rng.normal(size=(n, 3))
+    q = rng.normal(size=(n, 4))
+    s = rng.normal(-1, 0.5, size=(n, 3))
+    o = rng.normal(size=(n, 1)) + 2
+    geo, l2w = O.snapshot(mean, q, s, o, 0.01, 2.0)
+    R = torch_ref.build_rotation(torch.tensor(q / np.linalg.norm(q, axis=1, keepdims=True)))
+    clip = geo[:, 13].astype(np.float64)
+    A = R.numpy() * (np.exp(s) * clip[:, None])[:, None, :]
+    assert np.abs(l2w[:, :9].reshape(n, 3, 3) - A).max() < 1e-5 * np.abs(A).max()
+    M = geo[:, :9].reshape(n, 3, 3).astype(np.float64)
+    eye = np.einsum("nij,njk->nik", M, A)
+    assert np.abs(eye - np.eye(3)).max() < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# known-answer scenes of the reference tests (restated)
+# ---------------------------------------------------------------------------
+def mock_camera_rays(oracle_lib, n=32):
+    c2w = np.diag([-1.0, 1.0, 1.0]).astype(np.float32)  # -R with column 0 negated, R = -I
+    return O.pinhole_rays(n, n, 1.5, np.array([0.0, 0.0, 1.0], np.float32), c2w)
+
+
+def target_image():
+    from PIL import Image
+    import os
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "target_1.png")
+    return np.asarray(Image.open(path).convert("RGB"), dtype=np.float32) / 255.0  # [H,W,3]
+
+
+def run_case(oracle_lib, mean, log_scale, colors, rot, logit, bg, athr, tthr, p=2.0):
+    ro, rd = mock_camera_rays(oracle_lib)
+    geo, l2w = O.snapshot(mean, rot, log_scale, logit, athr, p)
+    k = O.make_knobs(exp_power=p, alpha_threshold=athr, t_threshold=tthr, bg=bg)
+    fwd = O.render(geo, colors, ro, rd, k, mode=0)
+    # float64 autograd restatement
+    t = lambda a: torch.tensor(np.asarray(a, np.float64), requires_grad=True)  # noqa: E731
+    tm, ts, tc, tq, to = t(mean), t(log_scale), t(colors), t(rot), t(logit)
+    img, extra = torch_ref.render(tm, tq, ts, to, tc, torch.tensor(ro, dtype=torch.float64),
+                                  torch.tensor(rd, dtype=torch.float64), p, athr, tthr, bg)
+    target = target_image().reshape(-1, 3)
+    gout = np.sign(img.detach().numpy() - target) / img.numel()
+    (img * torch.tensor(gout)).sum().backward()
+    bwd = O.backward(geo, l2w, colors, rot, log_scale, logit, ro, rd, fwd, gout.astype(np.float32), k, 32 * 32)
+    return fwd, img.detach().numpy(), bwd, {"mean": tm.grad, "rotation": tq.grad, "scale": ts.grad,
+                                            "opacity": to.grad, "rgb": tc.grad}
+
+
+def assert_grads_close(bwd, ref, tol=1e-4):
+    for key in ("mean", "rotation", "scale", "opacity", "rgb"):
+        a = bwd[key].reshape(-1)
+        b = ref[key].numpy().reshape(-1)
+        assert np.abs(a - b).max() < tol, (key, np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("athr", [0.01, 0.3])
+def test_single_gaussian_known_answer(oracle_lib, athr):
+    mean = np.array([[-0.05, -0.05, 0.0]])
+    log_scale = np.log([[0.1, 0.3, 0.3]])
+    colors = np.array([[1.0, 0.0, 0.0]])
+    rot = np.array([[-0.5, 0.0, 0.0, 1.0]])
+    logit = np.array([[0.0]])  # logit(0.5)
+    bg = (0.1, 0.1, 0.1)
+    fwd, ref_img, bwd, ref_g = run_case(oracle_lib, mean, log_scale, colors, rot, logit, bg, athr, 0.03)
+    # closed form C = alpha c + (1 - alpha) bg
+    alpha = (fwd["C"][:, 0] - 0.1) / 0.9
+    assert np.abs(fwd["C"][:, 1] - 0.1 * (1 - alpha)).max() < 1e-6
+    assert np.abs(fwd["C"] - ref_img).max() < 1e-5
+    assert np.abs(fwd["C"] - ref_img).mean() < 1e-6
+    assert (fwd["ncomp"] > 0).sum() > 20
+    assert_grads_close(bwd, ref_g)
+
+
+@pytest.mark.parametrize("tthr", [0.0, 0.5])
+def test_multiple_gaussians_known_answer(oracle_lib, tthr):
+    mean = np.array([[0.05, 0.05, -0.25], [-0.05, -0.05, 0.25], [0.0, 0.0, 0.0]])
+    log_scale = np.log([[1.0, 0.2, 0.1], [0.1, 0.3, 0.3], [0.4, 0.4, 0.4]])
+    colors = np.array([[0.0, 0.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    rot = np.array([[1.0, 0.0, 0.0, 1.0]] * 3)
+    logit = np.log(np.array([[0.5], [0.7], [0.7]]) / (1 - np.array([[0.5], [0.7], [0.7]])))
+    fwd, ref_img, bwd, ref_g = run_case(oracle_lib, mean, log_scale, colors, rot, logit, (0, 0, 0), 0.01, tthr)
+    assert np.abs(fwd["C"] - ref_img).max() < 1e-5
+    if tthr == 0.5:
+        assert (fwd["est"].sum(1) > 0).any(), "truncation estimate must be exercised"
+    assert_grads_close(bwd, ref_g)
+
+
+def random_small(seed, n=48, scale_mu=-1.6):
+    rng = np.random.default_rng(seed)
+    mean = rng.normal(0, 0.35, size=(n, 3))
+    q = rng.normal(size=(n, 4))
+    s = rng.normal(scale_mu, 0.4, size=(n, 3))
+    logit = rng.normal(size=(n, 1))
+    colors = rng.uniform(size=(n, 3))
+    return mean, q, s, logit, colors
+
+
+@pytest.mark.parametrize("seed,tthr,p", [(0, 0.03, 2.0), (1, 0.0, 2.0), (2, 0.3, 1.0), (3, 0.03, 3.0)])
+def test_random_scene_against_autograd(oracle_lib, seed, tthr, p):
+    mean, q, s, logit, colors = random_small(seed)
+    ro, rd = O.pinhole_rays(20, 20, 1.0, np.array([0, 0, 2.5], np.float32), np.diag([1.0, 1.0, 1.0]))
+    geo, l2w = O.snapshot(mean, q, s, logit, 0.01, p)
+    k = O.make_knobs(exp_power=p, alpha_threshold=0.01, t_threshold=tthr, bg=(0.2, 0.3, 0.1))
+    fwd = O.render(geo, colors, ro, rd, k, mode=0)
+    lit = O.render(geo, colors, ro, rd, k, mode=1)
+    # the restated truncation bookkeeping equals the literal reference arithmetic
+    assert np.abs(fwd["C"] - lit["C"]).max() < 2e-5
+    assert np.abs(fwd["F"] - lit["F"]).max() < 2e-6
+    t = lambda a: torch.tensor(np.asarray(a, np.float64), requires_grad=True)  # noqa: E731
+    tm, ts, tc, tq, to = t(mean), t(s), t(colors), t(q), t(logit)
+    img, extra = torch_ref.render(tm, tq, ts, to, tc, torch.tensor(ro, dtype=torch.float64),
+                                  torch.tensor(rd, dtype=torch.float64), p, 0.01, tthr, (0.2, 0.3, 0.1))
+    assert np.abs(fwd["C"] - img.detach().numpy()).max() < 5e-5
+    assert np.array_equal(fwd["ncomp"], np.array(extra["ncomp"]))
+    gout = np.random.default_rng(seed + 100).normal(size=img.shape) / img.numel()
+    (img * torch.tensor(gout)).sum().backward()
+    bwd = O.backward(geo, l2w, colors, q, s, logit, ro, rd, fwd, gout.astype(np.float32), k, 400.0)
+    for key, ref in (("mean", tm.grad), ("rotation", tq.grad), ("scale", ts.grad), ("opacity", to.grad),
+                     ("rgb", tc.grad)):
+        a, b = bwd[key].reshape(-1), ref.numpy().reshape(-1)
+        scale = max(np.abs(b).max(), 1e-8)
+        assert np.abs(a - b).max() < 1e-3 * scale + 1e-7, (key, np.abs(a - b).max(), scale)
+
+
+def test_invariants(oracle_lib):
+    mean, q, s, logit, colors = random_small(11, n=96)
+    ro, rd = O.pinhole_rays(16, 16, 1.0, np.array([0, 0, 2.5], np.float32), np.eye(3))
+    geo, _ = O.snapshot(mean, q, s, logit, 0.01, 2.0)
+    k = O.make_knobs(t_threshold=0.0)
+    fwd = O.render(geo, colors, ro, rd, k)
+    ids, dists, counts = O.hitlists(geo, ro, rd)
+    for r in range(ro.shape[0]):
+        n = fwd["ncomp"][r]
+        Tb = fwd["Tb"][r, :n]
+        assert np.all(np.diff(Tb) <= 0), "T must not increase along the composited list"
+        assert n == counts[r], "with t_threshold 0 every hit is composited"
+        assert np.all(np.diff(dists[r, :counts[r]]) >= 0)
+        assert list(fwd["ids"][r, :n]) == list(ids[r, :n])
+    # alpha range: every composited alpha in [MAX_ALPHA * athr * (1-1e-6), MAX_ALPHA]
+    # (checked through T ratios)
+    ratio = fwd["Tb"][:, 1:] / np.maximum(fwd["Tb"][:, :-1], 1e-30)
+    mask = np.arange(ratio.shape[1])[None, :] < (fwd["ncomp"][:, None] - 1)
+    a = 1 - ratio[mask]
+    assert a.min() >= MAX_ALPHA * 0.01 * (1 - 1e-3) and a.max() <= MAX_ALPHA * (1 + 1e-6)
+    # empty scene renders the background with T = F = 1
+    empty = O.render(np.zeros((0, 16), np.float32), np.zeros((0, 3), np.float32), ro, rd,
+                     O.make_knobs(bg=(0.25, 0.5, 0.75)))
+    assert np.allclose(empty["C"], [0.25, 0.5, 0.75]) and np.all(empty["T"] == 1) and np.all(empty["F"] == 1)
+
+
+def test_truncation_and_batches(oracle_lib):
+    # 80 coincident-axis Gaussians along the ray: several batches of 32, then the estimate
+    n = 80
+    mean = np.zeros((n, 3))
+    mean[:, 2] = -np.linspace(0.1, 4.0, n)
+    q = np.tile([1.0, 0, 0, 0], (n, 1))
+    s = np.full((n, 3), np.log(0.05))
+    logit = np.full((n, 1), -2.5)  # opacity ~0.076
+    colors = np.random.default_rng(0).uniform(size=(n, 3))
+    ro = np.zeros((1, 3), np.float32)
+    rd = np.array([[0, 0, -1.0]], np.float32)
+    geo, _ = O.snapshot(mean, q, s, logit, 0.01, 2.0)
+    for tthr in (0.0, 0.05):
+        fwd = O.render(geo, colors, ro, rd, O.make_knobs(t_threshold=tthr), mode=0)
+        lit = O.render(geo, colors, ro, rd, O.make_knobs(t_threshold=tthr), mode=1)
+        assert fwd["nhits"][0] == n
+        if tthr == 0.0:
+            assert fwd["ncomp"][0] == n
+        else:
+            assert 32 < fwd["ncomp"][0] < n and fwd["T"][0] < tthr
+        assert np.abs(fwd["C"] - lit["C"]).max() < 1e-5
+        # closed form along the axis: alpha_i = MAX_ALPHA * op, composite in order
+        op = 1 / (1 + np.exp(2.5))
+        a = MAX_ALPHA * op
+        T = 1.0
+        c = np.zeros(3)
+        k = 0
+        for i in range(n):
+            c += colors[i] * T * a
+            T *= 1 - a
+            k += 1
+            if T < tthr:
+                break
+        rest = colors[k:]
+        F = T * (1 - a) ** len(rest)
+        est = rest.mean(0) if len(rest) else np.zeros(3)
+        c += (T - F) * est
+        assert np.abs(fwd["C"][0] - c).max() < 1e-5
+
+
+def test_finite_differences_of_reference_formulation():
+    # pins the float64 restatement's autograd against central differences
+    mean, q, s, logit, colors = random_small(5, n=12, scale_mu=-1.2)
+    ro = torch.tensor([[0.05, -0.02, 2.5], [0.1, 0.1, 2.5], [-0.2, 0.05, 2.5]], dtype=torch.float64)
+    rd = torch.tensor([[0.0, 0.0, -1.0], [-0.03, -0.04, -1.0], [0.08, -0.02, -1.0]], dtype=torch.float64)
+    rd = rd / rd.norm(dim=1, keepdim=True)
+    params = [torch.tensor(a, dtype=torch.float64) for a in (mean, q, s, logit, colors)]
+
+    def f(ps):
+        img, _ = torch_ref.render(ps[0], ps[1], ps[2], ps[3], ps[4], ro, rd, 2.0, 0.01, 0.03, (0.1, 0.2, 0.3))
+        w = torch.linspace(0.5, 1.5, img.numel(), dtype=torch.float64).reshape(img.shape)
+        return (img * w).sum()
+
+    ps = [p.clone().requires_grad_() for p in params]
+    f(ps).backward()
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    checked = 0
+    for _ in range(40):
+        pi = int(rng.integers(0, 5))
+        idx = tuple(int(rng.integers(0, d)) for d in params[pi].shape)
+        if abs(float(ps[pi].grad[idx])) < 1e-9:
+            continue
+        plus = [p.clone() for p in params]
+        minus = [p.clone() for p in params]
+        plus[pi][idx] += h
+        minus[pi][idx] -= h
+        fd = (f(plus) - f(minus)).item() / (2 * h)
+        an = float(ps[pi].grad[idx])
+        assert abs(fd - an) < 1e-5 * max(1.0, abs(an)), (pi, idx, fd, an)
+        checked += 1
+    assert checked >= 10
