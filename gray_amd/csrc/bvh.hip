@@ -391,9 +391,14 @@ __global__ void k_karras(int n, const uint32_t *__restrict__ keys, const uint32_
     }
 }
 
+// N == 1: no sort runs, so the leaf order (sorted_ids, read by the refit) and
+// the root are written here.
 __global__ void k_single_leaf(BvhNode *__restrict__ nodes, uint32_t *__restrict__ parent_int,
                               uint32_t *__restrict__ parent_leaf, uint32_t *__restrict__ range_lo,
-                              uint32_t *__restrict__ range_hi) {
+                              uint32_t *__restrict__ range_hi, uint32_t *__restrict__ sorted_ids,
+                              uint32_t *__restrict__ keys) {
+    sorted_ids[0] = 0u;
+    keys[0] = 0u;
     BvhNode nd;
     box_empty(nd.lb);
     box_empty(nd.rb);
@@ -672,7 +677,7 @@ static int build_topology(gray_ctx *ctx, const float *mean, hipStream_t s) {
     const int n = (int)ctx->n;
     if (n == 1) {
         hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(1), 0, s, ctx->nodes, ctx->parent_int, ctx->parent_leaf,
-                           ctx->range_lo, ctx->range_hi);
+                           ctx->range_lo, ctx->range_hi, ctx->sorted_ids, ctx->keys);
         GRAY_HIP(hipGetLastError());
         ctx->num_top = 0;
         return GRAY_OK;

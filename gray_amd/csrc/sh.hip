@@ -265,7 +265,7 @@ extern "C" int gray_sh_backward(gray_ctx *ctx, const gray_params *params, const 
                                 const uint8_t *was_visible, float *grad_sh_dc, float *grad_sh_rest, float *grad_mean,
                                 int accumulate, void *stream) {
     if (!ctx || !params || !knobs || !knobs->enable_sh || !origins || !rgb || !grad_rgb || num_views <= 0 ||
-        !grad_sh_dc || !grad_sh_rest || !grad_mean) {
+        !grad_sh_dc || (params->num_sh_coeffs > 0 && !grad_sh_rest) || !grad_mean) {
         gray_set_error(GRAY_ERR_INVALID, "sh_backward: null argument or no views");
         return GRAY_ERR_INVALID;
     }

@@ -116,7 +116,14 @@ __device__ __forceinline__ void traverse(const TraceArgs &A, const BvhNode *__re
     }
     uint32_t node = 0;
     int sp = 0;
-    while (true) {
+    // every internal node is visited at most once: more iterations means a
+    // corrupted tree, reported instead of looping
+    const int max_iters = n + 16;
+    for (int it = 0;; ++it) {
+        if (it > max_iters) {
+            if (A.error_flag) atomicOr(A.error_flag, 1 << GRAY_ERR_STATE);
+            return;
+        }
         const BvhNode nd = nodes[node];
         const bool hl = part && nd.left != GRAY_EMPTY && slab_test(r, nd.lb, tlo);
         const bool hr = part && nd.right != GRAY_EMPTY && slab_test(r, nd.rb, tlo);

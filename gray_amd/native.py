@@ -160,17 +160,28 @@ class GaussianDataHolder:
         """Reference gaussians.h:198-263 (in-place resize_, storage kept)."""
         if self.count > 1 and n > self.count:
             raise RuntimeError("The new number of Gaussians must not be larger than the current count.")
+        old = self.count
         with torch.no_grad():
             self.count = int(n)
+            # resize_ keeps the prefix; elements it newly exposes are
+            # uninitialised memory, so they are cleared (learning rates copy [0])
             for name in self.PARAMS:
                 shp = self._shape(name, n)
-                getattr(self, name).resize_(shp)
-                for prefix in ("grad_", "first_moment_", "second_moment_"):
-                    getattr(self, prefix + name).resize_(shp)
+                for prefix in ("", "grad_", "first_moment_", "second_moment_"):
+                    t = getattr(self, prefix + name)
+                    t.resize_(shp)
+                    if n > old:
+                        t[old:].zero_()
             for name in self.LRS:
-                getattr(self, name).resize_(n, 1)
+                t = getattr(self, name)
+                t.resize_(n, 1)
+                if n > old:
+                    t[old:].copy_(t[:1].expand(n - old, 1))
             for name in ("pruning_weight", "pruning_counter", "was_visible", "marked"):
-                getattr(self, name).resize_(n, 1)
+                t = getattr(self, name)
+                t.resize_(n, 1)
+                if n > old:
+                    t[old:].zero_()
         self._attach_grads()
 
     def increment_sh_degree(self) -> None:
@@ -227,6 +238,45 @@ class HitStoreHolder:
         self.comp_rgb = torch.zeros(r, L.NUM_CHANNELS, dtype=F32, device=device)
 
 
+class PhaseTimer:
+    """HIP-event timing of the native calls, on the stream they are issued on
+    (torch's current stream).  Enabled by assigning NativeRaytracer.timer."""
+
+    def __init__(self):
+        self.events = {}
+
+    def start(self, name):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        return (name, ev)
+
+    def stop(self, token):
+        name, ev0 = token
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev1.record()
+        self.events.setdefault(name, []).append((ev0, ev1))
+
+    def reset(self):
+        self.events = {}
+
+    def summary(self):
+        """{phase: (launches, mean ms, total ms)} -- synchronises."""
+        torch.cuda.synchronize()
+        out = {}
+        for name, pairs in self.events.items():
+            ms = [a.elapsed_time(b) for a, b in pairs]
+            out[name] = (len(ms), sum(ms) / len(ms), sum(ms))
+        return out
+
+
+class _NoTimer:
+    def start(self, name):
+        return None
+
+    def stop(self, token):
+        pass
+
+
 def hit_capacity_for(pixels: int, ppll_backward_size: int, max_capacity: int = 64) -> int:
     """Stored hits per ray from the reference's backward-list budget
     (config.py:46-47, entries): a ray whose composited list is longer is
@@ -262,6 +312,7 @@ class NativeRaytracer:
         self.ppll_forward_size = int(ppll_forward_size)
         self.ppll_backward_size = int(ppll_backward_size)
         self.use_bvh = True
+        self.timer = _NoTimer()
         self._steps = 0
         self._built_count = -1
         self._forward_saved = False
@@ -326,8 +377,10 @@ class NativeRaytracer:
         g = self.gaussian_data
         origins = (C.c_void_p * 1)(self.camera_data.origin.data_ptr())
         outs = (C.c_void_p * 1)(g.channels.data_ptr())
+        tk = self.timer.start("sh_forward")
         L.check(L.LIB.gray_sh_forward(self.ctx.handle, C.byref(g.params()), C.byref(self.config_data.knobs()), 1,
                                       origins, outs, self._stream()))
+        self.timer.stop(tk)
 
     # ---- passes -----------------------------------------------------------
     def forward_pass(self) -> None:
@@ -338,7 +391,9 @@ class NativeRaytracer:
         self.gaussian_data.was_visible.fill_(False)
         self._sh_forward()
         view = self._view(save)
+        tk = self.timer.start("trace_forward")
         L.check(L.LIB.gray_render_forward(self.ctx.handle, C.byref(self._args(view, save)), self._stream()))
+        self.timer.stop(tk)
         self._forward_saved = save
 
     def forward_pass_display(self) -> None:
@@ -359,14 +414,18 @@ class NativeRaytracer:
         grads = L.Grads(g.grad_mean.data_ptr(), g.grad_rotation.data_ptr(), g.grad_scale.data_ptr(),
                         g.grad_opacity.data_ptr(), g.pruning_weight.data_ptr(), g.was_visible.data_ptr())
         params = g.params()
+        tk = self.timer.start("trace_backward")
         L.check(L.LIB.gray_render_backward(self.ctx.handle, C.byref(self._args(view, True)), C.byref(params),
                                            C.byref(grads), self._stream()))
+        self.timer.stop(tk)
         origins = (C.c_void_p * 1)(self.camera_data.origin.data_ptr())
         rgb = (C.c_void_p * 1)(g.channels.data_ptr())
         grgb = (C.c_void_p * 1)(g.grad_channels.data_ptr())
+        tk = self.timer.start("sh_backward")
         L.check(L.LIB.gray_sh_backward(self.ctx.handle, C.byref(params), C.byref(self.config_data.knobs()), 1,
                                        origins, rgb, grgb, g.was_visible.data_ptr(), g.grad_sh_coeffs_dc.data_ptr(),
                                        g.grad_sh_coeffs_rest.data_ptr(), g.grad_mean.data_ptr(), 0, self._stream()))
+        self.timer.stop(tk)
 
     def adam_struct(self, update_pruning_stats: bool) -> L.Adam:
         g, cfg = self.gaussian_data, self.config_data
@@ -400,7 +459,9 @@ class NativeRaytracer:
             self.meta_data.total_num_steps.add_(1)
         self._steps += 1
         a = self.adam_struct(update_pruning_stats)
+        tk = self.timer.start("adam")
         L.check(L.LIB.gray_adam_step(self.ctx.handle, C.byref(a), self._steps, 0, self._stream()))
+        self.timer.stop(tk)
 
     def update_bvh(self) -> None:
         """Reference raytracer.h:135 (BVHWrapper::update: snapshot + refit)."""
@@ -408,8 +469,10 @@ class NativeRaytracer:
             self.rebuild_bvh()
             return
         g = self.gaussian_data
+        tk = self.timer.start("bvh_refit")
         L.check(L.LIB.gray_bvh_update(self.ctx.handle, C.byref(g.params()), C.byref(self.config_data.knobs()), 0,
                                       self._stream()))
+        self.timer.stop(tk)
 
     def rebuild_bvh(self) -> None:
         """Reference raytracer.h:137-142 (full LBVH build)."""
