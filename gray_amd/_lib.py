@@ -106,6 +106,7 @@ def _load():
         "gray_adam_step": (i32, [vp, C.POINTER(Adam), i32, i32, vp]),
         "gray_debug_hitlists": (i32, [vp, C.POINTER(RenderArgs), i32, vp, vp, vp, vp]),
         "gray_debug_snapshot": (i32, [vp, vp, vp, vp]),
+        "gray_debug_bvh": (i32, [vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -119,7 +120,7 @@ EXPORTED = (
     "gray_ctx_create", "gray_ctx_destroy", "gray_last_error", "gray_abi_version", "gray_device_status",
     "gray_bvh_update", "gray_bvh_set_timing", "gray_bvh_last_build_ms", "gray_bvh_info", "gray_sh_forward",
     "gray_sh_backward", "gray_render_forward", "gray_render_backward", "gray_adam_step", "gray_debug_hitlists",
-    "gray_debug_snapshot",
+    "gray_debug_snapshot", "gray_debug_bvh",
 )
 
 

@@ -787,6 +787,18 @@ extern "C" int gray_bvh_info(gray_ctx *ctx, int64_t *num_gaussians, int64_t *num
     return GRAY_OK;
 }
 
+extern "C" int gray_debug_bvh(gray_ctx *ctx, float *nodes, uint32_t *sorted_ids, uint32_t *keys, void *stream) {
+    if (!ctx) return GRAY_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = ctx->n;
+    if (n == 0) return GRAY_OK;
+    const int64_t nn = n > 1 ? n - 1 : 1;
+    if (nodes) GRAY_HIP(hipMemcpyAsync(nodes, ctx->nodes, sizeof(BvhNode) * nn, hipMemcpyDeviceToDevice, s));
+    if (sorted_ids) GRAY_HIP(hipMemcpyAsync(sorted_ids, ctx->sorted_ids, 4 * n, hipMemcpyDeviceToDevice, s));
+    if (keys) GRAY_HIP(hipMemcpyAsync(keys, ctx->keys, 4 * n, hipMemcpyDeviceToDevice, s));
+    return GRAY_OK;
+}
+
 extern "C" int gray_debug_snapshot(gray_ctx *ctx, float *geo, float *aabb, void *stream) {
     if (!ctx) return GRAY_ERR_INVALID;
     hipStream_t s = (hipStream_t)stream;

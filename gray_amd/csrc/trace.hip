@@ -38,8 +38,10 @@ __device__ __forceinline__ bool slab_test(const LaneRay &r, const float *b, floa
     const float tz0 = (b[2] - r.oz) * r.iz, tz1 = (b[5] - r.oz) * r.iz;
     const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
     const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
-    // relative slack keeps the cull conservative under rounding
-    const float slack = 1e-5f * fmaxf(fabsf(tn), fabsf(tf)) + 1e-7f;
+    // relative slack keeps the cull conservative under rounding; clamped so
+    // an infinite slab bound (a zero direction component) cannot make it
+    // infinite and accept every box
+    const float slack = 1e-5f * fminf(fmaxf(fabsf(tn), fabsf(tf)), 1e30f) + 1e-7f;
     return (b[0] <= b[3]) && (tn <= tf + slack) && (tf + slack >= tlo);
 }
 

@@ -537,6 +537,18 @@ class NativeRaytracer:
         L.check(L.LIB.gray_debug_snapshot(self.ctx.handle, geo.data_ptr(), aabb.data_ptr(), self._stream()))
         return geo, aabb
 
+    def debug_bvh(self):
+        """LBVH of the last rebuild: nodes [N-1,16] float32 (refs as int bits),
+        sorted_ids [N], keys [N]."""
+        n = self.gaussian_data.count
+        nn = max(n - 1, 1)
+        nodes = torch.zeros(nn, 16, dtype=F32, device=self.device)
+        ids = torch.zeros(n, dtype=torch.int32, device=self.device)
+        keys = torch.zeros(n, dtype=torch.int32, device=self.device)
+        L.check(L.LIB.gray_debug_bvh(self.ctx.handle, nodes.data_ptr(), ids.data_ptr(), keys.data_ptr(),
+                                     self._stream()))
+        return nodes, ids, keys
+
     def debug_hitlists(self, max_hits: int, use_bvh: bool = True):
         """Every hit of every ray of the current camera, sorted by (distance,
         id): ids [R, max_hits], dists [R, max_hits], counts [R], R = w*h."""

@@ -252,6 +252,13 @@ int gray_debug_hitlists(gray_ctx *ctx, const gray_render_args *args, int32_t max
  * aabb [N,6] = {min xyz, max xyz}.  Either pointer may be NULL. */
 int gray_debug_snapshot(gray_ctx *ctx, float *geo, float *aabb, void *stream);
 
+/* No reference counterpart (OptiX's acceleration structure is opaque).
+ * Copies the LBVH (device buffers): nodes [max(N-1,1), 16] floats =
+ * {left box 6, right box 6, left ref, right ref, range lo, range hi} with refs
+ * as uint32 bits (bit 31 = leaf, payload Gaussian id; 0xffffffff = none),
+ * sorted_ids [N] (leaf order), keys [N] (30-bit Morton codes, ascending). */
+int gray_debug_bvh(gray_ctx *ctx, float *nodes, uint32_t *sorted_ids, uint32_t *keys, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
