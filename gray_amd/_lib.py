@@ -61,7 +61,7 @@ class View(C.Structure):
 class RenderArgs(C.Structure):
     _fields_ = [
         ("knobs", Knobs), ("num_views", i32), ("views", C.POINTER(View)), ("save", i32),
-        ("hit_capacity", i32), ("use_bvh", i32),
+        ("hit_capacity", i32), ("use_bvh", i32), ("tile_stats", vp), ("list_capacity", i32),
     ]
 
 
@@ -107,6 +107,7 @@ def _load():
         "gray_debug_hitlists": (i32, [vp, C.POINTER(RenderArgs), i32, vp, vp, vp, vp]),
         "gray_debug_snapshot": (i32, [vp, vp, vp, vp]),
         "gray_debug_bvh": (i32, [vp, vp, vp, vp, vp]),
+        "gray_debug_list_counts": (i32, [vp, vp, i64, C.POINTER(i64), vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -115,12 +116,16 @@ def _load():
     return lib
 
 
+ABI_VERSION = 2  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
 LIB = _load()
+if LIB.gray_abi_version() != ABI_VERSION:
+    raise ImportError(f"libgray_hip.so has ABI {LIB.gray_abi_version()}, the binding expects {ABI_VERSION}; "
+                      "rebuild with python -m gray_amd.build")
 EXPORTED = (
     "gray_ctx_create", "gray_ctx_destroy", "gray_last_error", "gray_abi_version", "gray_device_status",
     "gray_bvh_update", "gray_bvh_set_timing", "gray_bvh_last_build_ms", "gray_bvh_info", "gray_sh_forward",
     "gray_sh_backward", "gray_render_forward", "gray_render_backward", "gray_adam_step", "gray_debug_hitlists",
-    "gray_debug_snapshot", "gray_debug_bvh",
+    "gray_debug_snapshot", "gray_debug_bvh", "gray_debug_list_counts",
 )
 
 

@@ -14,8 +14,13 @@
 
 #define GRAY_WAVE 64
 #define GRAY_TILE 8             // 8x8 rays per wave64
-#define GRAY_KB GRAY_BATCH      // k-buffer entries per ray (reference BUFFER_SIZE)
-#define GRAY_STACK 128          // traversal stack entries per wave (LDS)
+#define GRAY_KB 16              // k-buffer entries per ray; divides the reference batch (32)
+// Traversal stack entries per lane (LDS).  Every internal node of the Karras
+// tree splits its range on a strictly longer common prefix of the 30-bit
+// Morton code extended by the 32-bit index, so the depth is at most
+// 30 + ceil(log2 N) + 1 and a near-first stack never holds more than depth - 1
+// entries: 64 suffices for N < 2^33.
+#define GRAY_STACK 64
 #define GRAY_MAX_VIEWS 8        // views per launch (kernel-argument table)
 #define GRAY_LEAF 0x80000000u   // child reference flag: leaf, payload = Gaussian id
 #define GRAY_EMPTY 0xFFFFFFFFu  // child reference: nothing
@@ -79,6 +84,16 @@ struct TraceArgs {
     int32_t *dbg_count;
     int32_t dbg_max, pad2;
     int32_t *error_flag;       // set to a GRAY_ERR_* code on device-side failure
+    int32_t *tile_stats;       // optional [tiles, 4]: node visits, leaf tests, rounds, hits
+    int32_t tile_base_launch, pad3;  // first tile index of this launch (views beyond 8 are launched in chunks)
+    // hit lists of the collect pass, slot-major: entry k of lane g at k*list_stride+g,
+    // g = launch tile * 64 + lane; list_n[g] = hits, -1 = incomplete (re-traverse)
+    float *list_d;
+    uint32_t *list_id;
+    float *list_a;
+    int32_t *list_n;
+    int64_t list_stride;
+    int32_t list_cap, pad4;
 };
 
 // ---------------------------------------------------------------------------

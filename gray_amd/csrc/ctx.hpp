@@ -37,6 +37,14 @@ struct gray_ctx {
     uint32_t *top_count = nullptr;  // [1] device
     int64_t num_top = 0;
     int32_t *err = nullptr;         // [1] device error flag
+    // hit lists of the forward collect pass (one launch of <= 8 views)
+    float *list_d = nullptr;
+    uint32_t *list_id = nullptr;
+    float *list_a = nullptr;
+    int32_t *list_n = nullptr;
+    int64_t list_entries = 0;       // allocated entries of list_d/list_id/list_a
+    int64_t list_lanes_cap = 0;     // allocated entries of list_n
+    int64_t list_lanes = 0;         // lanes of the last collecting launch
     // timing
     bool timing = false;
     float last_build_ms = -1.0f;

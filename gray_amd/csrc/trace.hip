@@ -6,8 +6,9 @@
 // the slab / exact ellipsoid test for its own ray, and a ballot decides the
 // descent.  Every lane keeps its GRAY_KB nearest hits (distance, id, alpha)
 // sorted in LDS; compositing follows the reference batch semantics
-// (forward_pass.cu:150-230): each round takes the 32 nearest hits strictly
-// beyond tmin, composites front to back and stops at T < t_threshold.
+// (forward_pass.cu:150-230): each batch takes the 32 nearest hits strictly
+// beyond tmin (served in sub-rounds of GRAY_KB), composites front to back and
+// stops at T < t_threshold.
 //
 // Truncation bookkeeping.  The reference estimates the colour of the hits it
 // never composited as (sum_all a*c - sum_comp a*c) / (sum_all a - sum_comp a)
@@ -47,10 +48,11 @@ __device__ __forceinline__ bool slab_test(const LaneRay &r, const float *b, floa
 
 // XCD-aware block -> tile mapping: workgroup b runs on XCD (b % 8); give each
 // XCD a contiguous band of tiles so neighbouring tiles share that XCD's L2.
-__device__ __forceinline__ bool block_tile(const TraceArgs &A, int &view, int &tx, int &ty) {
+__device__ __forceinline__ bool block_tile(const TraceArgs &A, int &view, int &tx, int &ty, int *tile_out = nullptr) {
     const int b = blockIdx.x;
     const int tile = (b & 7) * A.xcd_chunk + (b >> 3);
     if (tile >= A.total_tiles) return false;
+    if (tile_out) *tile_out = tile;
     int v = 0;
 #pragma unroll
     for (int k = 1; k < GRAY_MAX_VIEWS; ++k)
@@ -103,70 +105,156 @@ __device__ __forceinline__ bool make_ray(const DevView &V, int x, int y, LaneRay
     return l2 != 0.0f;
 }
 
-// Packet traversal.  Calls leaf(gid) (wave-uniform gid) for every Gaussian
-// whose box some participating lane's ray interval [tlo, inf) overlaps, or for
-// every Gaussian when use_bvh == 0.  Must be called by the whole wave.
+// Wave-uniform node and Gaussian records are fetched with broadcast vector
+// loads (every lane reads the same 64 bytes; the memory system serves one line
+// per wave).  Vector loads complete in order (vmcnt), so a prefetched record
+// does not force a wait on the LDS traffic of the k-buffer the way an
+// out-of-order scalar load (lgkmcnt) would.
+struct NodeV {
+    float lb[6], rb[6];
+    uint32_t left, right;
+};
+
+struct GeoV {
+    float m[9], mu[3], op, clip;
+};
+
+__device__ __forceinline__ uint32_t in_vgpr(uint32_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+__device__ __forceinline__ NodeV load_node(const BvhNode *__restrict__ nodes, uint32_t idx) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + in_vgpr(idx));
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+    NodeV n;
+    n.lb[0] = a.x, n.lb[1] = a.y, n.lb[2] = a.z, n.lb[3] = a.w, n.lb[4] = b.x, n.lb[5] = b.y;
+    n.rb[0] = b.z, n.rb[1] = b.w, n.rb[2] = c.x, n.rb[3] = c.y, n.rb[4] = c.z, n.rb[5] = c.w;
+    n.left = __float_as_uint(d.x);
+    n.right = __float_as_uint(d.y);
+    return n;
+}
+
+__device__ __forceinline__ GeoV load_geo(const GeoRecord *__restrict__ geo, uint32_t gid) {
+    const float4 *p = reinterpret_cast<const float4 *>(geo + in_vgpr(gid));
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3];
+    GeoV g;
+    g.m[0] = a.x, g.m[1] = a.y, g.m[2] = a.z, g.m[3] = a.w, g.m[4] = b.x, g.m[5] = b.y, g.m[6] = b.z;
+    g.m[7] = b.w, g.m[8] = c.x, g.mu[0] = c.y, g.mu[1] = c.z, g.mu[2] = c.w, g.op = d.x, g.clip = d.y;
+    return g;
+}
+
+__device__ __forceinline__ void load_rgb(const float *__restrict__ rgb, uint32_t gid, float *c) {
+    const float *q = rgb + 3 * (size_t)in_vgpr(gid);
+    c[0] = q[0];
+    c[1] = q[1];
+    c[2] = q[2];
+}
+
+__device__ __forceinline__ float uniform_f(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Per-ray traversal.  Every lane walks the LBVH for its own ray with its own
+// stack ([level][lane] in LDS, near child first along the ray); the wave
+// loops until every lane is done (while-while).  In one iteration a lane
+// visits one node, then tests the (up to two) leaves among that node's hit
+// children: their records are requested before the next node is, so the
+// node load overlaps the leaf tests.  Calls leaf(gid, record, c0, c1, c2) for
+// every Gaussian whose box the lane's ray interval [tlo, inf) overlaps (or for
+// every Gaussian when use_bvh == 0), c = colour when want_rgb.  Must be
+// called by the whole wave.
 template <typename Leaf>
-__device__ __forceinline__ void traverse(const TraceArgs &A, const BvhNode *__restrict__ nodes, const LaneRay &r,
-                                         bool part, float tlo, uint32_t *__restrict__ stack, float cdx, float cdy,
-                                         float cdz, Leaf &&leaf) {
+__device__ __forceinline__ void traverse(const TraceArgs &A, const BvhNode *__restrict__ nodes,
+                                         const GeoRecord *__restrict__ geo, const float *__restrict__ rgb,
+                                         bool want_rgb, const LaneRay &r, bool part, float tlo,
+                                         uint32_t *__restrict__ stack, int lane, Leaf &&leaf,
+                                         int *visits = nullptr) {
     const int n = A.num_gaussians;
     if (n <= 0) return;
     if (!A.use_bvh) {
-        for (int g = 0; g < n; ++g) leaf((uint32_t)g);
+        for (int g = 0; g < n; ++g) {
+            const GeoV rec = load_geo(geo, (uint32_t)g);
+            float c[3] = {0.0f, 0.0f, 0.0f};
+            if (want_rgb) load_rgb(rgb, (uint32_t)g, c);
+            if (part) leaf((uint32_t)g, rec, c[0], c[1], c[2]);
+        }
         return;
     }
-    uint32_t node = 0;
+    uint32_t node = part ? 0u : GRAY_EMPTY;
+    NodeV nd;
+    if (node != GRAY_EMPTY) nd = load_node(nodes, node);
     int sp = 0;
-    // every internal node is visited at most once: more iterations means a
-    // corrupted tree, reported instead of looping
-    const int max_iters = n + 16;
+    const int max_iters = 2 * n + 16;  // a lane visits each node at most once
     for (int it = 0;; ++it) {
+        if (!__ballot(node != GRAY_EMPTY)) break;
         if (it > max_iters) {
             if (A.error_flag) atomicOr(A.error_flag, 1 << GRAY_ERR_STATE);
             return;
         }
-        const BvhNode nd = nodes[node];
-        const bool hl = part && nd.left != GRAY_EMPTY && slab_test(r, nd.lb, tlo);
-        const bool hr = part && nd.right != GRAY_EMPTY && slab_test(r, nd.rb, tlo);
-        const uint64_t ml = __ballot(hl), mr = __ballot(hr);
-        uint32_t next = GRAY_EMPTY, other = GRAY_EMPTY;
-        if (ml) {
-            if (nd.left & GRAY_LEAF)
-                leaf(nd.left & ~GRAY_LEAF);
-            else
-                next = nd.left;
-        }
-        if (mr) {
-            if (nd.right & GRAY_LEAF)
-                leaf(nd.right & ~GRAY_LEAF);
-            else if (next == GRAY_EMPTY)
-                next = nd.right;
-            else
-                other = nd.right;
-        }
-        if (other != GRAY_EMPTY) {
-            // near child first along the tile's central direction
-            const float cl = ((nd.lb[0] + nd.lb[3]) * cdx + (nd.lb[1] + nd.lb[4]) * cdy) + (nd.lb[2] + nd.lb[5]) * cdz;
-            const float cr = ((nd.rb[0] + nd.rb[3]) * cdx + (nd.rb[1] + nd.rb[4]) * cdy) + (nd.rb[2] + nd.rb[5]) * cdz;
-            if (__builtin_amdgcn_readfirstlane((int)(cr < cl))) {
-                const uint32_t t = next;
-                next = other;
-                other = t;
+        uint32_t la = GRAY_EMPTY, lb = GRAY_EMPTY, next = GRAY_EMPTY;
+        if (node != GRAY_EMPTY) {
+            if (visits) ++visits[0];
+            const bool hl = nd.left != GRAY_EMPTY && slab_test(r, nd.lb, tlo);
+            const bool hr = nd.right != GRAY_EMPTY && slab_test(r, nd.rb, tlo);
+            uint32_t other = GRAY_EMPTY;
+            if (hl) {
+                if (nd.left & GRAY_LEAF)
+                    la = nd.left & ~GRAY_LEAF;
+                else
+                    next = nd.left;
             }
-            if (sp >= GRAY_STACK) {
-                if (A.error_flag) atomicOr(A.error_flag, 1 << GRAY_ERR_OVERFLOW);
-                return;
+            if (hr) {
+                if (nd.right & GRAY_LEAF)
+                    lb = nd.right & ~GRAY_LEAF;
+                else if (next == GRAY_EMPTY)
+                    next = nd.right;
+                else
+                    other = nd.right;
             }
-            stack[sp] = other;
-            ++sp;
+            if (other != GRAY_EMPTY) {
+                const float cl = ((nd.lb[0] + nd.lb[3]) * r.dx + (nd.lb[1] + nd.lb[4]) * r.dy) +
+                                 (nd.lb[2] + nd.lb[5]) * r.dz;
+                const float cr = ((nd.rb[0] + nd.rb[3]) * r.dx + (nd.rb[1] + nd.rb[4]) * r.dy) +
+                                 (nd.rb[2] + nd.rb[5]) * r.dz;
+                if (cr < cl) {
+                    const uint32_t t = next;
+                    next = other;
+                    other = t;
+                }
+                if (sp < GRAY_STACK) {
+                    stack[sp * GRAY_WAVE + lane] = other;
+                    ++sp;
+                } else if (A.error_flag) {
+                    atomicOr(A.error_flag, 1 << GRAY_ERR_OVERFLOW);
+                }
+            }
+            if (next == GRAY_EMPTY && sp > 0) {
+                --sp;
+                next = stack[sp * GRAY_WAVE + lane];
+            }
         }
-        if (next == GRAY_EMPTY) {
-            if (sp == 0) break;
-            --sp;
-            node = (uint32_t)__builtin_amdgcn_readfirstlane((int)stack[sp]);
-        } else {
-            node = next;
+        // leaf records first, then the next node: the leaf tests wait only on
+        // the former (vector loads complete in order)
+        GeoV ga, gb;
+        float ca[3] = {0.0f, 0.0f, 0.0f}, cb[3] = {0.0f, 0.0f, 0.0f};
+        if (la != GRAY_EMPTY) {
+            ga = load_geo(geo, la);
+            if (want_rgb) load_rgb(rgb, la, ca);
+        }
+        if (lb != GRAY_EMPTY) {
+            gb = load_geo(geo, lb);
+            if (want_rgb) load_rgb(rgb, lb, cb);
+        }
+        node = next;
+        if (node != GRAY_EMPTY) nd = load_node(nodes, node);
+        if (la != GRAY_EMPTY) {
+            if (visits) ++visits[1];
+            leaf(la, ga, ca[0], ca[1], ca[2]);
+        }
+        if (lb != GRAY_EMPTY) {
+            if (visits) ++visits[1];
+            leaf(lb, gb, cb[0], cb[1], cb[2]);
         }
     }
 }
@@ -219,36 +307,159 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
     return res;
 }
 
-__device__ __forceinline__ void tile_central_dir(const DevView &V, int tx, int ty, const LaneRay &r, bool active,
-                                                 float &cx, float &cy, float &cz) {
-    // direction of the first active lane (all rays of a tile are close)
-    const uint64_t m = __ballot(active);
-    if (!m) {
-        cx = cy = cz = 0.0f;
-        return;
+// ---------------------------------------------------------------------------
+// Collect pass: every hit of every ray into its list
+// ---------------------------------------------------------------------------
+// Stack entries per lane of the collect pass (LDS).  A lane that would need
+// more falls back to re-traversal, so the bound costs speed, never results.
+constexpr int kCollectStack = 32;
+
+// Each lane walks the LBVH for its own ray with two nodes in flight per
+// iteration (the order of the walk does not matter here: the list is
+// composited in distance order afterwards).  In an iteration the lane tests
+// the children of both nodes, requests the records of the hit leaves, then
+// the next two nodes, and runs the exact hit test on the leaves while the node
+// records are in flight.  Every hit (distance, id, alpha) goes to the lane's
+// list; a lane whose list or stack overflows stops and is marked -1.
+__global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRecord *__restrict__ geo,
+                                                       const BvhNode *__restrict__ nodes) {
+    __shared__ uint32_t s_stack[kCollectStack * GRAY_WAVE];
+    int vi, tx, ty, tile;
+    if (!block_tile(A, vi, tx, ty, &tile)) return;
+    const DevView &V = A.views[vi];
+    const int lane = threadIdx.x;
+    const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
+    LaneRay r{};
+    bool active = false;
+    if (x < V.width && y < V.height) active = make_ray(V, x, y, r);
+    const size_t g = (size_t)tile * GRAY_WAVE + lane;
+    const size_t S = (size_t)A.list_stride;
+    const int cap = A.list_cap;
+    const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power;
+    float *__restrict__ out_d = A.list_d;
+    uint32_t *__restrict__ out_id = A.list_id;
+    float *__restrict__ out_a = A.list_a;
+
+    int cnt = 0, visits = 0, leaves = 0;
+    bool ok = true;
+    uint32_t h0 = (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY, h1 = GRAY_EMPTY;
+    NodeV n0, n1;
+    if (h0 != GRAY_EMPTY) n0 = load_node(nodes, h0);
+    int sp = 0;
+    for (;;) {
+        // invariant: h1 holds a node only when h0 does
+        if (!__ballot(h0 != GRAY_EMPTY)) break;
+        uint32_t lf[4] = {GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY};
+        uint32_t in[4] = {GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY};
+        if (h0 != GRAY_EMPTY) {
+            ++visits;
+            if (n0.left != GRAY_EMPTY && slab_test(r, n0.lb, 0.0f))
+                (n0.left & GRAY_LEAF ? lf[0] : in[0]) = n0.left;
+            if (n0.right != GRAY_EMPTY && slab_test(r, n0.rb, 0.0f))
+                (n0.right & GRAY_LEAF ? lf[1] : in[1]) = n0.right;
+        }
+        if (h1 != GRAY_EMPTY) {
+            ++visits;
+            if (n1.left != GRAY_EMPTY && slab_test(r, n1.lb, 0.0f))
+                (n1.left & GRAY_LEAF ? lf[2] : in[2]) = n1.left;
+            if (n1.right != GRAY_EMPTY && slab_test(r, n1.rb, 0.0f))
+                (n1.right & GRAY_LEAF ? lf[3] : in[3]) = n1.right;
+        }
+        // next two nodes: hit internal children first, the rest onto the stack
+        uint32_t nh0 = GRAY_EMPTY, nh1 = GRAY_EMPTY;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (in[k] == GRAY_EMPTY) continue;
+            if (nh0 == GRAY_EMPTY) {
+                nh0 = in[k];
+            } else if (nh1 == GRAY_EMPTY) {
+                nh1 = in[k];
+            } else if (sp < kCollectStack) {
+                s_stack[sp * GRAY_WAVE + lane] = in[k];
+                ++sp;
+            } else {
+                ok = false;
+            }
+        }
+        if (nh0 == GRAY_EMPTY && sp > 0) {
+            --sp;
+            nh0 = s_stack[sp * GRAY_WAVE + lane];
+        }
+        if (nh1 == GRAY_EMPTY && sp > 0) {
+            --sp;
+            nh1 = s_stack[sp * GRAY_WAVE + lane];
+        }
+        if (!ok) nh0 = nh1 = GRAY_EMPTY;
+        GeoV ga[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (lf[k] != GRAY_EMPTY) ga[k] = load_geo(geo, lf[k] & ~GRAY_LEAF);
+        h0 = nh0;
+        h1 = nh1;
+        if (h0 != GRAY_EMPTY) n0 = load_node(nodes, h0);
+        if (h1 != GRAY_EMPTY) n1 = load_node(nodes, h1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (lf[k] == GRAY_EMPTY) continue;
+            ++leaves;
+            const GeoV &q = ga[k];
+            if (q.op <= athr) continue;
+            float dist, hx, hy, hz;
+            if (!gray_hit_test(q.m, q.mu[0], q.mu[1], q.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz))
+                continue;
+            float gv, qq;
+            const float a = gray_alpha(hx, hy, hz, q.clip, q.op, p, gv, qq);
+            if (cnt < cap) {
+                const size_t e = (size_t)cnt * S + g;
+                out_d[e] = dist;
+                out_id[e] = lf[k] & ~GRAY_LEAF;
+                out_a[e] = a;
+            }
+            ++cnt;
+        }
+        if (cnt > cap) ok = false;
+        if (!ok) {
+            h0 = h1 = GRAY_EMPTY;
+            sp = 0;
+        }
     }
-    const int src = __ffsll((long long)m) - 1;
-    cx = __shfl(r.dx, src);
-    cy = __shfl(r.dy, src);
-    cz = __shfl(r.dz, src);
-    cx = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cx)));
-    cy = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cy)));
-    cz = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(cz)));
+    A.list_n[g] = ok ? cnt : -1;
+    if (A.tile_stats) {
+        int hsum = ok ? cnt : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            hsum += __shfl_xor(hsum, off);
+            visits += __shfl_xor(visits, off);
+            leaves += __shfl_xor(leaves, off);
+        }
+        if (lane == 0) {
+            int32_t *ts = A.tile_stats + 4 * (size_t)(A.tile_base_launch + tile);
+            ts[0] = visits;
+            ts[1] = leaves;
+            ts[3] = hsum;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------
+// Hit sources of the forward: the BVH (every round re-traverses) or the list
+// the collect pass wrote.  With lists on, the list kernel takes the lanes
+// whose list is complete and the traversal kernel the others.
+enum ForwardSource { kFromBvh = 0, kFromList = 1 };
+
+template <int SRC>
 __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const GeoRecord *__restrict__ geo,
                                                              const BvhNode *__restrict__ nodes,
                                                              const float *__restrict__ rgb0) {
     __shared__ float s_kd[GRAY_KB * GRAY_WAVE];
     __shared__ uint32_t s_ki[GRAY_KB * GRAY_WAVE];
     __shared__ float s_ka[GRAY_KB * GRAY_WAVE];
-    __shared__ uint32_t s_stack[GRAY_STACK];
+    __shared__ uint32_t s_stack[SRC == kFromBvh ? GRAY_STACK * GRAY_WAVE : 1];
 
-    int vi, tx, ty;
-    if (!block_tile(A, vi, tx, ty)) return;
+    int vi, tx, ty, tile;
+    if (!block_tile(A, vi, tx, ty, &tile)) return;
     const DevView &V = A.views[vi];
     const int lane = threadIdx.x;
     const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
@@ -256,8 +467,13 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
     LaneRay r{};
     bool active = false;
     if (inside) active = make_ray(V, x, y, r);
-    float cdx, cdy, cdz;
-    tile_central_dir(V, tx, ty, r, active, cdx, cdy, cdz);
+    const bool lists = A.list_cap > 0;
+    const size_t lg = (size_t)tile * GRAY_WAVE + lane;
+    const int lcount = lists ? A.list_n[lg] : -1;
+    if (lists) active = active && (SRC == kFromList ? lcount >= 0 : lcount < 0);
+    if (!__ballot(active)) return;
+    int visits[2] = {0, 0};
+    int nrounds = 0;
 
     const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power, tthr = *A.knobs.t_threshold;
     const float znear = *V.znear, zfar = *V.zfar;
@@ -271,36 +487,30 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
     float skc0 = 0.0f, skc1 = 0.0f, skc2 = 0.0f, ska = 0.0f, pnc = 1.0f;
     float tmin = znear;
     uint32_t ncomp = 0, nhits = 0;
-    bool alive = active;
+    bool alive = active && tmin < zfar;
+    // Round cursor: a reference batch (32 hits strictly beyond tmin,
+    // forward_pass.cu:150-182) is served in sub-rounds of GRAY_KB; inside a
+    // batch the next sub-round continues after the last (dist, id) key, at a
+    // batch boundary it restarts strictly beyond tmin.
+    float cur_d = znear;
+    uint32_t cur_id = 0;
+    bool strict = true;
+    int bpos = 0, batch = 0;
 
-    for (int round = 0; round < GRAY_MAX_ROUNDS; ++round) {
-        const bool part = alive && tmin < zfar;
+    for (int round = 0; round < 2 * GRAY_MAX_ROUNDS * (GRAY_BATCH / GRAY_KB); ++round) {
+        const bool part = alive;
         if (!__ballot(part)) break;
         int count = 0;
         float lastd = 0.0f;
         uint32_t lasti = 0;
         const bool first = round == 0;
-        auto leaf = [&](uint32_t gid) {
-            const GeoRecord rec = geo[gid];
-            if (rec.op <= athr) return;
-            float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-            if (first) {
-                c0 = rgb[3 * gid + 0];
-                c1 = rgb[3 * gid + 1];
-                c2 = rgb[3 * gid + 2];
-            }
-            if (!part) return;
-            float dist, hx, hy, hz;
-            if (!gray_hit_test(rec.m, rec.mu[0], rec.mu[1], rec.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx,
-                               hy, hz))
-                return;
-            float gv, q;
-            const float a = gray_alpha(hx, hy, hz, rec.clip, rec.op, p, gv, q);
+        auto found = [&](uint32_t gid, float dist, float a, float c0, float c1, float c2) {
             if (first) ++nhits;
+            const bool beyond = strict ? dist > cur_d : (dist > cur_d || (dist == cur_d && gid > cur_id));
             int res = 0;
             uint32_t ev_id = 0;
             float ev_a = 0.0f;
-            if (dist > tmin) res = kb_insert(kb, count, lastd, lasti, dist, gid, a, ev_id, ev_a);
+            if (beyond) res = kb_insert(kb, count, lastd, lasti, dist, gid, a, ev_id, ev_a);
             if (!first) return;
             if (res == 0) {
                 skc0 += a * c0;
@@ -316,7 +526,37 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
                 pnc = pnc * (1.0f - ev_a);
             }
         };
-        traverse(A, nodes, r, part, first ? 0.0f : tmin, s_stack, cdx, cdy, cdz, leaf);
+        if constexpr (SRC == kFromList) {
+            if (part) {
+                const size_t S = (size_t)A.list_stride;
+                for (int i = 0; i < lcount; ++i) {
+                    const size_t e = (size_t)i * S + lg;
+                    const uint32_t gid = A.list_id[e];
+                    const float dist = A.list_d[e], a = A.list_a[e];
+                    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
+                    if (first) {
+                        c0 = rgb[3 * gid + 0];
+                        c1 = rgb[3 * gid + 1];
+                        c2 = rgb[3 * gid + 2];
+                    }
+                    found(gid, dist, a, c0, c1, c2);
+                }
+            }
+        } else {
+            auto leaf = [&](uint32_t gid, const GeoV &g, float c0, float c1, float c2) {
+                if (g.op <= athr) return;
+                float dist, hx, hy, hz;
+                if (!gray_hit_test(g.m, g.mu[0], g.mu[1], g.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy,
+                                   hz))
+                    return;
+                float gv, q;
+                const float a = gray_alpha(hx, hy, hz, g.clip, g.op, p, gv, q);
+                found(gid, dist, a, c0, c1, c2);
+            };
+            traverse(A, nodes, geo, rgb, first, r, part, first ? 0.0f : cur_d, s_stack, lane, leaf,
+                     A.tile_stats ? visits : nullptr);
+        }
+        ++nrounds;
 
         if (part) {
             bool terminated = false;
@@ -367,11 +607,46 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
                     pnc = pnc * (1.0f - a);
                 }
             }
-            if (!terminated && count < GRAY_KB) tmin = FLT_MAX;  // empty slots (reference: FLT_MAX)
-            alive = !terminated && count == GRAY_KB && tmin < zfar;
+            alive = false;
+            if (!terminated) {
+                if (count < GRAY_KB) {
+                    tmin = FLT_MAX;  // a partial batch ends the loop (reference: FLT_MAX slots)
+                } else {
+                    bpos += GRAY_KB;
+                    if (bpos >= GRAY_BATCH) {
+                        bpos = 0;
+                        strict = true;
+                        cur_d = tmin;
+                        ++batch;
+                    } else {
+                        strict = false;
+                        cur_d = kb.dist(GRAY_KB - 1);
+                        cur_id = kb.gid(GRAY_KB - 1);
+                    }
+                    alive = tmin < zfar && batch < GRAY_MAX_ROUNDS;
+                }
+            }
         }
     }
 
+    if (A.tile_stats) {
+        int hsum = (int)nhits;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            hsum += __shfl_xor(hsum, off);
+            visits[0] += __shfl_xor(visits[0], off);
+            visits[1] += __shfl_xor(visits[1], off);
+        }
+        if (lane == 0) {
+            int32_t *ts = A.tile_stats + 4 * (size_t)(A.tile_base_launch + tile);
+            if (SRC == kFromList || !lists) ts[2] = nrounds;  // visits, leaves, hits: collect pass
+            if (!lists) {
+                ts[0] = visits[0];
+                ts[1] = visits[1];
+                ts[3] = hsum;
+            }
+        }
+    }
     if (!active) return;
     const float F = T * pnc;
     const float rem = T - F;
@@ -583,7 +858,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
     __shared__ float s_kd[GRAY_KB * GRAY_WAVE];
     __shared__ uint32_t s_ki[GRAY_KB * GRAY_WAVE];
     __shared__ float s_ka[GRAY_KB * GRAY_WAVE];
-    __shared__ uint32_t s_stack[GRAY_STACK];
+    __shared__ uint32_t s_stack[GRAY_STACK * GRAY_WAVE];
     int vi, tx, ty;
     if (!block_tile(A, vi, tx, ty)) return;
     const DevView &V = A.views[vi];
@@ -598,8 +873,6 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
     LaneRay r{};
     PixelState ps{};
     if (need) need = load_pixel_state(A, V, pix, r, ps);
-    float cdx, cdy, cdz;
-    tile_central_dir(V, tx, ty, r, need, cdx, cdy, cdz);
     const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power, tthr = *A.knobs.t_threshold;
     const float eps_sg = *A.knobs.eps_scale_grad;
     const float znear = *V.znear, zfar = *V.zfar;
@@ -609,32 +882,41 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
         Atot = (ps.g0 * V.comp_rgb[3 * pix + 0] + ps.g1 * V.comp_rgb[3 * pix + 1]) + ps.g2 * V.comp_rgb[3 * pix + 2];
     float P = 0.0f, T = 1.0f, tmin = znear;
     int done = 0;
-    bool alive = need;
-    for (int round = 0; round < GRAY_MAX_ROUNDS; ++round) {
-        const bool part = alive && tmin < zfar;
+    bool alive = need && tmin < zfar;
+    // same round cursor as the forward
+    float cur_d = znear;
+    uint32_t cur_id = 0;
+    bool strict = true;
+    int bpos = 0, batch = 0;
+    for (int round = 0; round < 2 * GRAY_MAX_ROUNDS * (GRAY_BATCH / GRAY_KB); ++round) {
+        const bool part = alive;
         if (!__ballot(part)) break;
         int count = 0;
         float lastd = 0.0f;
         uint32_t lasti = 0;
-        auto leaf = [&](uint32_t gid) {
-            const GeoRecord rec = geo[gid];
-            if (rec.op <= athr) return;
-            if (!part) return;
+        auto leaf = [&](uint32_t gid, const GeoV &g, float, float, float) {
+            if (g.op <= athr) return;
             float dist, hx, hy, hz;
-            if (!gray_hit_test(rec.m, rec.mu[0], rec.mu[1], rec.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx,
-                               hy, hz))
+            if (!gray_hit_test(g.m, g.mu[0], g.mu[1], g.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz))
                 return;
-            if (!(dist > tmin)) return;
+            const bool beyond = strict ? dist > cur_d : (dist > cur_d || (dist == cur_d && gid > cur_id));
+            if (!beyond) return;
             float gv, q;
-            const float a = gray_alpha(hx, hy, hz, rec.clip, rec.op, p, gv, q);
+            const float a = gray_alpha(hx, hy, hz, g.clip, g.op, p, gv, q);
             uint32_t ev_id;
             float ev_a;
             kb_insert(kb, count, lastd, lasti, dist, gid, a, ev_id, ev_a);
         };
-        traverse(A, nodes, r, part, tmin, s_stack, cdx, cdy, cdz, leaf);
+        traverse(A, nodes, geo, V.rgb, false, r, part, cur_d, s_stack, lane, leaf);
         if (part) {
             bool terminated = false;
             int used = 0;
+            float last_d = 0.0f;
+            uint32_t last_id = 0;
+            if (count == GRAY_KB) {
+                last_d = kb.dist(GRAY_KB - 1);
+                last_id = kb.gid(GRAY_KB - 1);
+            }
             for (int k = 0; k < count; ++k) {
                 const float d = kb.dist(k);
                 const uint32_t gid = kb.gid(k);
@@ -660,7 +942,6 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
                     kb.id[k * GRAY_WAVE + lane] = GRAY_EMPTY;
                 }
             }
-            if (!terminated && count < GRAY_KB) tmin = FLT_MAX;
             const bool last = done >= n;
             float gT = last ? 0.0f : (Atot - P) / T;
             for (int k = used - 1; k >= 0; --k) {
@@ -668,7 +949,21 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
                 if (gid == GRAY_EMPTY) continue;
                 backward_hit(A, V, r, p, eps_sg, ps, gid, kb.alpha(k), gT);
             }
-            alive = !last && !terminated && count == GRAY_KB && tmin < zfar;
+            alive = false;
+            if (!terminated && !last && count == GRAY_KB) {
+                bpos += GRAY_KB;
+                if (bpos >= GRAY_BATCH) {
+                    bpos = 0;
+                    strict = true;
+                    cur_d = tmin;
+                    ++batch;
+                } else {
+                    strict = false;
+                    cur_d = last_d;
+                    cur_id = last_id;
+                }
+                alive = tmin < zfar && batch < GRAY_MAX_ROUNDS;
+            }
         }
     }
 }
@@ -678,7 +973,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(GRAY_WAVE) void k_debug_hits(TraceArgs A, const GeoRecord *__restrict__ geo,
                                                           const BvhNode *__restrict__ nodes) {
-    __shared__ uint32_t s_stack[GRAY_STACK];
+    __shared__ uint32_t s_stack[GRAY_STACK * GRAY_WAVE];
     int vi, tx, ty;
     if (!block_tile(A, vi, tx, ty)) return;
     if (vi != 0) return;
@@ -689,18 +984,13 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_debug_hits(TraceArgs A, const Geo
     LaneRay r{};
     bool active = false;
     if (inside) active = make_ray(V, x, y, r);
-    float cdx, cdy, cdz;
-    tile_central_dir(V, tx, ty, r, active, cdx, cdy, cdz);
     const float athr = *A.knobs.alpha_threshold;
     const int ray = y * V.width + x;
     int cnt = 0;
-    auto leaf = [&](uint32_t gid) {
-        const GeoRecord rec = geo[gid];
-        if (rec.op <= athr) return;
-        if (!active) return;
+    auto leaf = [&](uint32_t gid, const GeoV &g, float, float, float) {
+        if (g.op <= athr) return;
         float dist, hx, hy, hz;
-        if (!gray_hit_test(rec.m, rec.mu[0], rec.mu[1], rec.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy,
-                           hz))
+        if (!gray_hit_test(g.m, g.mu[0], g.mu[1], g.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz))
             return;
         if (cnt < A.dbg_max) {
             A.dbg_ids[(size_t)ray * A.dbg_max + cnt] = gid;
@@ -708,7 +998,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_debug_hits(TraceArgs A, const Geo
         }
         ++cnt;
     };
-    traverse(A, nodes, r, active, 0.0f, s_stack, cdx, cdy, cdz, leaf);
+    traverse(A, nodes, geo, V.rgb, false, r, active, 0.0f, s_stack, lane, leaf);
     if (inside) A.dbg_count[ray] = active ? cnt : 0;
 }
 
@@ -800,10 +1090,47 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
     A.capacity = args->hit_capacity < 0 ? 0 : args->hit_capacity;
     A.use_bvh = args->use_bvh;
     A.num_gaussians = (int)ctx->n;
+    A.tile_stats = args->tile_stats;
+    A.tile_base_launch = 0;
     A.geo = ctx->geo;
     A.nodes = ctx->nodes;
     A.l2w = ctx->l2w;
     A.error_flag = ctx->err;
+    return GRAY_OK;
+}
+
+// Grows the collect-pass lists to lanes * cap entries (never shrinks).
+int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
+    if (cap > (1 << 16)) {
+        gray_set_error(GRAY_ERR_INVALID, "render: list_capacity %d exceeds 65536", cap);
+        return GRAY_ERR_INVALID;
+    }
+    const int64_t entries = lanes * (int64_t)cap;
+    if (lanes > ctx->list_lanes_cap) {
+        if (ctx->list_n) GRAY_HIP(hipFree(ctx->list_n));
+        ctx->list_n = nullptr;
+        ctx->list_lanes_cap = 0;
+        GRAY_HIP(hipMalloc(&ctx->list_n, sizeof(int32_t) * lanes));
+        ctx->list_lanes_cap = lanes;
+    }
+    if (entries > ctx->list_entries) {
+        void *bufs[3] = {ctx->list_d, ctx->list_id, ctx->list_a};
+        for (void *b : bufs)
+            if (b) GRAY_HIP(hipFree(b));
+        ctx->list_d = nullptr;
+        ctx->list_id = nullptr;
+        ctx->list_a = nullptr;
+        ctx->list_entries = 0;
+        if (hipMalloc(&ctx->list_d, sizeof(float) * entries) != hipSuccess ||
+            hipMalloc(&ctx->list_id, sizeof(uint32_t) * entries) != hipSuccess ||
+            hipMalloc(&ctx->list_a, sizeof(float) * entries) != hipSuccess) {
+            (void)hipGetLastError();
+            gray_set_error(GRAY_ERR_NOMEM, "render: cannot allocate hit lists (%lld entries of 12 bytes)",
+                           (long long)entries);
+            return GRAY_ERR_NOMEM;
+        }
+        ctx->list_entries = entries;
+    }
     return GRAY_OK;
 }
 
@@ -825,10 +1152,13 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
     int rc = check_render(ctx, args);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
+    int tiles_before = 0;
     for (int first = 0; first < args->num_views; first += GRAY_MAX_VIEWS) {
         const int nv = std::min(GRAY_MAX_VIEWS, args->num_views - first);
         TraceArgs A;
         if ((rc = fill_args(ctx, args, first, nv, A))) return rc;
+        A.tile_base_launch = tiles_before;
+        tiles_before += A.total_tiles;
         if (A.save) {
             for (int i = 0; i < nv; ++i) {
                 const DevView &d = A.views[i];
@@ -841,10 +1171,43 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             }
         }
         if (A.total_tiles == 0) continue;
-        hipLaunchKernelGGL(k_trace_forward, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, A.geo, A.nodes,
-                           A.views[0].rgb);
+        const dim3 grid(8 * A.xcd_chunk), block(GRAY_WAVE);
+        if (args->list_capacity > 0 && A.use_bvh && A.num_gaussians > 0) {
+            const int64_t lanes = (int64_t)A.total_tiles * GRAY_WAVE;
+            if ((rc = reserve_lists(ctx, lanes, args->list_capacity))) return rc;
+            A.list_d = ctx->list_d;
+            A.list_id = ctx->list_id;
+            A.list_a = ctx->list_a;
+            A.list_n = ctx->list_n;
+            A.list_stride = lanes;
+            A.list_cap = args->list_capacity;
+            ctx->list_lanes = lanes;
+            hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, A.nodes);
+            GRAY_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_trace_forward<kFromList>, grid, block, 0, s, A, A.geo, A.nodes, A.views[0].rgb);
+            GRAY_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_trace_forward<kFromBvh>, grid, block, 0, s, A, A.geo, A.nodes, A.views[0].rgb);
         GRAY_HIP(hipGetLastError());
     }
+    return GRAY_OK;
+}
+
+extern "C" int gray_debug_list_counts(gray_ctx *ctx, int32_t *counts, int64_t max_lanes, int64_t *lanes,
+                                      void *stream) {
+    if (!ctx || !lanes) {
+        gray_set_error(GRAY_ERR_INVALID, "debug_list_counts: null context or lanes pointer");
+        return GRAY_ERR_INVALID;
+    }
+    *lanes = ctx->list_lanes;
+    if (!counts || ctx->list_lanes == 0) return GRAY_OK;
+    if (max_lanes < ctx->list_lanes) {
+        gray_set_error(GRAY_ERR_INVALID, "debug_list_counts: %lld entries needed, room for %lld",
+                       (long long)ctx->list_lanes, (long long)max_lanes);
+        return GRAY_ERR_INVALID;
+    }
+    GRAY_HIP(hipMemcpyAsync(counts, ctx->list_n, sizeof(int32_t) * ctx->list_lanes, hipMemcpyDeviceToDevice,
+                            (hipStream_t)stream));
     return GRAY_OK;
 }
 

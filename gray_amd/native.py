@@ -21,6 +21,9 @@ import torch
 from . import _lib as L
 
 F32 = torch.float32
+# hits per ray kept by the forward's collect pass (12 bytes each): 512 covers
+# the grazing rays of dense surface scenes; longer lists re-traverse
+DEFAULT_LIST_CAPACITY = 512
 
 
 def _device() -> torch.device:
@@ -312,6 +315,10 @@ class NativeRaytracer:
         self.ppll_forward_size = int(ppll_forward_size)
         self.ppll_backward_size = int(ppll_backward_size)
         self.use_bvh = True
+        # hits collected per ray before compositing (0: composite while
+        # traversing); rays with more hits are re-traversed per batch
+        self.list_capacity = DEFAULT_LIST_CAPACITY
+        self.tile_stats = None  # optional int32 [tiles, 4] traversal statistics of the forward
         self.timer = _NoTimer()
         self._steps = 0
         self._built_count = -1
@@ -367,6 +374,8 @@ class NativeRaytracer:
         a.save = 1 if save else 0
         a.hit_capacity = self.hits.capacity
         a.use_bvh = 1 if self.use_bvh else 0
+        a.tile_stats = self.tile_stats.data_ptr() if self.tile_stats is not None else None
+        a.list_capacity = int(self.list_capacity)
         return a
 
     def _require_bvh(self) -> None:
@@ -548,6 +557,16 @@ class NativeRaytracer:
         L.check(L.LIB.gray_debug_bvh(self.ctx.handle, nodes.data_ptr(), ids.data_ptr(), keys.data_ptr(),
                                      self._stream()))
         return nodes, ids, keys
+
+    def debug_list_counts(self):
+        """Hits per ray of the last forward that collected lists, int32
+        [tiles*64] in tile order (lane = 8*(y%8) + x%8; -1 = re-traversed)."""
+        n = C.c_int64(0)
+        L.check(L.LIB.gray_debug_list_counts(self.ctx.handle, None, 0, C.byref(n), self._stream()))
+        out = torch.zeros(max(n.value, 1), dtype=torch.int32, device=self.device)
+        L.check(L.LIB.gray_debug_list_counts(self.ctx.handle, out.data_ptr(), out.numel(), C.byref(n),
+                                             self._stream()))
+        return out[:n.value]
 
     def debug_hitlists(self, max_hits: int, use_bvh: bool = True):
         """Every hit of every ray of the current camera, sorted by (distance,

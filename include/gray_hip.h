@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 1
+#define GRAY_ABI_VERSION 2
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -117,6 +117,13 @@ typedef struct {
     int32_t save;               /* forward: write backward state + ray outputs */
     int32_t hit_capacity;       /* L: hits stored per ray (0 = recompute all in backward) */
     int32_t use_bvh;            /* 1 = BVH traversal, 0 = test every Gaussian (brute force) */
+    int32_t *tile_stats;        /* optional [tiles, 4] traversal statistics per 8x8 tile of the forward:
+                                   node visits, leaf tests, rounds, hits (summed over the tile's rays);
+                                   tiles are numbered view by view, row-major.  NULL = off */
+    int32_t list_capacity;      /* hits collected per ray by the forward's traversal pass before
+                                   compositing (0 = composite during traversal, re-traversing per batch).
+                                   Rays with more hits fall back to re-traversal.  The context keeps
+                                   list_capacity * 12 bytes per ray of one launch (<= 8 views). */
 } gray_render_args;
 
 /* Gradient targets of the backward (reference gaussians.h:34-62). */
@@ -258,6 +265,13 @@ int gray_debug_snapshot(gray_ctx *ctx, float *geo, float *aabb, void *stream);
  * as uint32 bits (bit 31 = leaf, payload Gaussian id; 0xffffffff = none),
  * sorted_ids [N] (leaf order), keys [N] (30-bit Morton codes, ascending). */
 int gray_debug_bvh(gray_ctx *ctx, float *nodes, uint32_t *sorted_ids, uint32_t *keys, void *stream);
+
+/* No reference counterpart (diagnostic of the collect pass).  Copies the hit
+ * counts per ray of the last forward launch that collected lists (device
+ * buffer, int32 [tiles*64], tile-major in the launch's tile order, lane =
+ * 8*(y%8) + x%8; -1 = the ray fell back to re-traversal).  *lanes receives
+ * the number of entries; counts may be NULL to query it. */
+int gray_debug_list_counts(gray_ctx *ctx, int32_t *counts, int64_t max_lanes, int64_t *lanes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,8 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
     lib.gray_abi_version.restype = ctypes.c_int
-    assert lib.gray_abi_version() == 1
+    m = re.search(r"#define GRAY_ABI_VERSION (\d+)", open(HEADER).read())
+    assert lib.gray_abi_version() == int(m.group(1))
     lib.gray_last_error.restype = ctypes.c_char_p
     assert isinstance(lib.gray_last_error(), bytes)
 

@@ -15,12 +15,15 @@ from gpu_util import mock_camera, rays_of, rel_err, setup_native
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(scope="module")
-def tiny():
+# hit sources of the forward: collected lists (default), composite while
+# traversing (0), and lists so short that most rays of a tile fall back to
+# re-traversal while their neighbours composite from lists (3)
+@pytest.fixture(scope="module", params=[None, 0, 3], ids=["lists", "traverse", "mixed"])
+def tiny(request):
     torch.manual_seed(0)
     scene = scenes.tiny_scene(1024, seed=0)
     cam = scenes.tiny_camera(64, 64)
-    nr = setup_native(scene, cam)
+    nr = setup_native(scene, cam, list_capacity=request.param)
     torch.set_grad_enabled(True)
     nr.forward_pass()
     torch.cuda.synchronize()
@@ -121,6 +124,23 @@ def test_forward_matches_oracle(tiny):
     orgb = O.sh_forward(scene.mean, scene.sh_dc, scene.sh_rest, cam.origin.astype(np.float32), 3)
     full = O.render(ogeo, orgb, oro, ord_, k)
     assert np.abs(img - full["C"]).max() < 1e-4
+
+
+def test_list_counts_match_oracle(tiny):
+    scene, cam, nr = tiny
+    if nr.list_capacity == 0:
+        assert nr.debug_list_counts().numel() == 0
+        return
+    counts = nr.debug_list_counts().cpu().numpy().reshape(8, 8, 8, 8)  # tile y, tile x, lane y, lane x
+    counts = counts.transpose(0, 2, 1, 3).reshape(-1)
+    geo, _ = nr.debug_snapshot()
+    ro, rd = rays_of(nr)
+    same = O.render(geo.cpu().numpy(), nr.get_gaussians().channels.cpu().numpy(), ro, rd, oracle_knobs(nr))
+    complete = counts >= 0
+    assert np.array_equal(counts[complete], same["nhits"][complete])
+    assert np.all(same["nhits"][~complete] > nr.list_capacity)
+    if nr.list_capacity < 8:
+        assert 0 < complete.sum() < complete.size
 
 
 def test_backward_matches_oracle(tiny):
