@@ -313,17 +313,24 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 // Stack entries per lane of the collect pass (LDS).  A lane that would need
 // more falls back to re-traversal, so the bound costs speed, never results.
 constexpr int kCollectStack = 32;
+// Leaf queue per lane (LDS ring): leaves found by node visits wait here for
+// their exact test, one per iteration.
+constexpr int kLeafQueue = 8;
 
-// Each lane walks the LBVH for its own ray with two nodes in flight per
-// iteration (the order of the walk does not matter here: the list is
-// composited in distance order afterwards).  In an iteration the lane tests
-// the children of both nodes, requests the records of the hit leaves, then
-// the next two nodes, and runs the exact hit test on the leaves while the node
-// records are in flight.  Every hit (distance, id, alpha) goes to the lane's
-// list; a lane whose list or stack overflows stops and is marked -1.
+// Each lane walks the LBVH for its own ray (the order of the walk does not
+// matter here: the list is composited in distance order afterwards).  One
+// iteration of a lane: the exact test of one queued leaf, whose record was
+// requested in the previous iteration, and the box tests of the children of
+// up to two nodes; hit leaves join the queue, hit internal children become
+// the next two nodes or go onto the stack.  Queuing keeps the expensive leaf
+// test dense across the wave (a node visit finds a leaf only about one time
+// in four); node visits pause while the queue could not take four more
+// leaves.  Every hit (distance, id, alpha) goes to the lane's list; a lane
+// whose list or stack overflows stops and is marked -1.
 __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRecord *__restrict__ geo,
                                                        const BvhNode *__restrict__ nodes) {
     __shared__ uint32_t s_stack[kCollectStack * GRAY_WAVE];
+    __shared__ uint32_t s_leaf[kLeafQueue * GRAY_WAVE];
     int vi, tx, ty, tile;
     if (!block_tile(A, vi, tx, ty, &tile)) return;
     const DevView &V = A.views[vi];
@@ -345,82 +352,89 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
     uint32_t h0 = (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY, h1 = GRAY_EMPTY;
     NodeV n0, n1;
     if (h0 != GRAY_EMPTY) n0 = load_node(nodes, h0);
-    int sp = 0;
+    uint32_t pl = GRAY_EMPTY;  // leaf whose record pg is in flight; GRAY_EMPTY implies an empty queue
+    GeoV pg;
+    int sp = 0, qh = 0, qn = 0;
     for (;;) {
-        // invariant: h1 holds a node only when h0 does
-        if (!__ballot(h0 != GRAY_EMPTY)) break;
-        uint32_t lf[4] = {GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY};
-        uint32_t in[4] = {GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY};
-        if (h0 != GRAY_EMPTY) {
-            ++visits;
-            if (n0.left != GRAY_EMPTY && slab_test(r, n0.lb, 0.0f))
-                (n0.left & GRAY_LEAF ? lf[0] : in[0]) = n0.left;
-            if (n0.right != GRAY_EMPTY && slab_test(r, n0.rb, 0.0f))
-                (n0.right & GRAY_LEAF ? lf[1] : in[1]) = n0.right;
-        }
-        if (h1 != GRAY_EMPTY) {
-            ++visits;
-            if (n1.left != GRAY_EMPTY && slab_test(r, n1.lb, 0.0f))
-                (n1.left & GRAY_LEAF ? lf[2] : in[2]) = n1.left;
-            if (n1.right != GRAY_EMPTY && slab_test(r, n1.rb, 0.0f))
-                (n1.right & GRAY_LEAF ? lf[3] : in[3]) = n1.right;
-        }
-        // next two nodes: hit internal children first, the rest onto the stack
-        uint32_t nh0 = GRAY_EMPTY, nh1 = GRAY_EMPTY;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (in[k] == GRAY_EMPTY) continue;
-            if (nh0 == GRAY_EMPTY) {
-                nh0 = in[k];
-            } else if (nh1 == GRAY_EMPTY) {
-                nh1 = in[k];
-            } else if (sp < kCollectStack) {
-                s_stack[sp * GRAY_WAVE + lane] = in[k];
-                ++sp;
-            } else {
-                ok = false;
-            }
-        }
-        if (nh0 == GRAY_EMPTY && sp > 0) {
-            --sp;
-            nh0 = s_stack[sp * GRAY_WAVE + lane];
-        }
-        if (nh1 == GRAY_EMPTY && sp > 0) {
-            --sp;
-            nh1 = s_stack[sp * GRAY_WAVE + lane];
-        }
-        if (!ok) nh0 = nh1 = GRAY_EMPTY;
-        GeoV ga[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (lf[k] != GRAY_EMPTY) ga[k] = load_geo(geo, lf[k] & ~GRAY_LEAF);
-        h0 = nh0;
-        h1 = nh1;
-        if (h0 != GRAY_EMPTY) n0 = load_node(nodes, h0);
-        if (h1 != GRAY_EMPTY) n1 = load_node(nodes, h1);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (lf[k] == GRAY_EMPTY) continue;
+        // invariants: h1 holds a node only when h0 does; pl is empty only when the queue is
+        if (!__ballot(h0 != GRAY_EMPTY || pl != GRAY_EMPTY)) break;
+        if (pl != GRAY_EMPTY) {
             ++leaves;
-            const GeoV &q = ga[k];
-            if (q.op <= athr) continue;
-            float dist, hx, hy, hz;
-            if (!gray_hit_test(q.m, q.mu[0], q.mu[1], q.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz))
-                continue;
-            float gv, qq;
-            const float a = gray_alpha(hx, hy, hz, q.clip, q.op, p, gv, qq);
-            if (cnt < cap) {
-                const size_t e = (size_t)cnt * S + g;
-                out_d[e] = dist;
-                out_id[e] = lf[k] & ~GRAY_LEAF;
-                out_a[e] = a;
+            if (pg.op > athr) {
+                float dist, hx, hy, hz;
+                if (gray_hit_test(pg.m, pg.mu[0], pg.mu[1], pg.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx,
+                                  hy, hz)) {
+                    float gv, qq;
+                    const float a = gray_alpha(hx, hy, hz, pg.clip, pg.op, p, gv, qq);
+                    if (cnt < cap) {
+                        const size_t e = (size_t)cnt * S + g;
+                        out_d[e] = dist;
+                        out_id[e] = pl;
+                        out_a[e] = a;
+                    }
+                    ++cnt;
+                }
             }
-            ++cnt;
+        }
+        uint32_t nh0 = h0, nh1 = h1;
+        bool moved = false;
+        if (h0 != GRAY_EMPTY && qn <= kLeafQueue - 4) {
+            moved = true;
+            uint32_t ch[4] = {GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY};
+            ++visits;
+            if (n0.left != GRAY_EMPTY && slab_test(r, n0.lb, 0.0f)) ch[0] = n0.left;
+            if (n0.right != GRAY_EMPTY && slab_test(r, n0.rb, 0.0f)) ch[1] = n0.right;
+            if (h1 != GRAY_EMPTY) {
+                ++visits;
+                if (n1.left != GRAY_EMPTY && slab_test(r, n1.lb, 0.0f)) ch[2] = n1.left;
+                if (n1.right != GRAY_EMPTY && slab_test(r, n1.rb, 0.0f)) ch[3] = n1.right;
+            }
+            nh0 = nh1 = GRAY_EMPTY;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t c = ch[k];
+                if (c == GRAY_EMPTY) continue;
+                if (c & GRAY_LEAF) {
+                    s_leaf[((qh + qn) & (kLeafQueue - 1)) * GRAY_WAVE + lane] = c & ~GRAY_LEAF;
+                    ++qn;
+                } else if (nh0 == GRAY_EMPTY) {
+                    nh0 = c;
+                } else if (nh1 == GRAY_EMPTY) {
+                    nh1 = c;
+                } else if (sp < kCollectStack) {
+                    s_stack[sp * GRAY_WAVE + lane] = c;
+                    ++sp;
+                } else {
+                    ok = false;
+                }
+            }
+            if (nh0 == GRAY_EMPTY && sp > 0) {
+                --sp;
+                nh0 = s_stack[sp * GRAY_WAVE + lane];
+            }
+            if (nh1 == GRAY_EMPTY && sp > 0) {
+                --sp;
+                nh1 = s_stack[sp * GRAY_WAVE + lane];
+            }
         }
         if (cnt > cap) ok = false;
         if (!ok) {
-            h0 = h1 = GRAY_EMPTY;
-            sp = 0;
+            nh0 = nh1 = GRAY_EMPTY;
+            sp = qn = 0;
+        }
+        // requests: the next leaf record first, then the next nodes
+        pl = GRAY_EMPTY;
+        if (qn > 0) {
+            pl = s_leaf[qh * GRAY_WAVE + lane];
+            qh = (qh + 1) & (kLeafQueue - 1);
+            --qn;
+            pg = load_geo(geo, pl);
+        }
+        h0 = nh0;
+        h1 = nh1;
+        if (moved) {
+            if (h0 != GRAY_EMPTY) n0 = load_node(nodes, h0);
+            if (h1 != GRAY_EMPTY) n1 = load_node(nodes, h1);
         }
     }
     A.list_n[g] = ok ? cnt : -1;
@@ -441,22 +455,74 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
     }
 }
 
-// ---------------------------------------------------------------------------
-// Forward
-// ---------------------------------------------------------------------------
-// Hit sources of the forward: the BVH (every round re-traverses) or the list
-// the collect pass wrote.  With lists on, the list kernel takes the lanes
-// whose list is complete and the traversal kernel the others.
-enum ForwardSource { kFromBvh = 0, kFromList = 1 };
+// Pixel outputs of the forward (reference forward_pass.cu:232-268): colour
+// with the truncation estimate and background, depth, and the backward state.
+// skc/ska/pnc are the skip-set sums (sum a*c, sum a, prod (1-a)).
+__device__ __forceinline__ void store_pixel(const TraceArgs &A, const DevView &V, int pix, const LaneRay &r,
+                                            bool depth_on, float T, float pnc, float skc0, float skc1, float skc2,
+                                            float ska, float C0, float C1, float C2, float D, uint32_t ncomp,
+                                            uint32_t nhits) {
+    const float F = T * pnc;
+    const float rem = T - F;
+    float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
+    if (ska > 0.0f) {
+        e0 = skc0 / ska;
+        e1 = skc1 / ska;
+        e2 = skc2 / ska;
+    }
+    const float cc0 = C0, cc1 = C1, cc2 = C2;
+    C0 += rem * e0;
+    C1 += rem * e1;
+    C2 += rem * e2;
+    if (depth_on) D = D / fmaxf(1.0f - T, *A.knobs.eps_fwd);
+    const float *bg = A.knobs.bg;
+    C0 += bg[0] * F;
+    C1 += bg[1] * F;
+    C2 += bg[2] * F;
+    V.out_c[3 * pix + 0] = C0;
+    V.out_c[3 * pix + 1] = C1;
+    V.out_c[3 * pix + 2] = C2;
+    if (V.out_depth) V.out_depth[pix] = D;
+    if (A.save) {
+        if (V.out_T) V.out_T[pix] = T;
+        if (V.out_F) V.out_F[pix] = F;
+        if (V.out_est) {
+            V.out_est[3 * pix + 0] = e0;
+            V.out_est[3 * pix + 1] = e1;
+            V.out_est[3 * pix + 2] = e2;
+        }
+        if (V.out_ro) {
+            V.out_ro[3 * pix + 0] = r.ox;
+            V.out_ro[3 * pix + 1] = r.oy;
+            V.out_ro[3 * pix + 2] = r.oz;
+        }
+        if (V.out_rd) {
+            V.out_rd[3 * pix + 0] = r.dx;
+            V.out_rd[3 * pix + 1] = r.dy;
+            V.out_rd[3 * pix + 2] = r.dz;
+        }
+        if (V.n_comp) V.n_comp[pix] = ncomp;
+        if (V.comp_rgb) {
+            V.comp_rgb[3 * pix + 0] = cc0;
+            V.comp_rgb[3 * pix + 1] = cc1;
+            V.comp_rgb[3 * pix + 2] = cc2;
+        }
+        if (V.stat_hits) V.stat_hits[pix] += (int)nhits;
+        if (V.stat_acc) V.stat_acc[pix] += (int)ncomp;
+    }
+}
 
-template <int SRC>
+// ---------------------------------------------------------------------------
+// Forward by re-traversal: every round walks the BVH again.  With lists on it
+// takes only the lanes whose list overflowed.
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const GeoRecord *__restrict__ geo,
                                                              const BvhNode *__restrict__ nodes,
                                                              const float *__restrict__ rgb0) {
     __shared__ float s_kd[GRAY_KB * GRAY_WAVE];
     __shared__ uint32_t s_ki[GRAY_KB * GRAY_WAVE];
     __shared__ float s_ka[GRAY_KB * GRAY_WAVE];
-    __shared__ uint32_t s_stack[SRC == kFromBvh ? GRAY_STACK * GRAY_WAVE : 1];
+    __shared__ uint32_t s_stack[GRAY_STACK * GRAY_WAVE];
 
     int vi, tx, ty, tile;
     if (!block_tile(A, vi, tx, ty, &tile)) return;
@@ -470,7 +536,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
     const bool lists = A.list_cap > 0;
     const size_t lg = (size_t)tile * GRAY_WAVE + lane;
     const int lcount = lists ? A.list_n[lg] : -1;
-    if (lists) active = active && (SRC == kFromList ? lcount >= 0 : lcount < 0);
+    if (lists) active = active && lcount < 0;
     if (!__ballot(active)) return;
     int visits[2] = {0, 0};
     int nrounds = 0;
@@ -526,23 +592,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
                 pnc = pnc * (1.0f - ev_a);
             }
         };
-        if constexpr (SRC == kFromList) {
-            if (part) {
-                const size_t S = (size_t)A.list_stride;
-                for (int i = 0; i < lcount; ++i) {
-                    const size_t e = (size_t)i * S + lg;
-                    const uint32_t gid = A.list_id[e];
-                    const float dist = A.list_d[e], a = A.list_a[e];
-                    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f;
-                    if (first) {
-                        c0 = rgb[3 * gid + 0];
-                        c1 = rgb[3 * gid + 1];
-                        c2 = rgb[3 * gid + 2];
-                    }
-                    found(gid, dist, a, c0, c1, c2);
-                }
-            }
-        } else {
+        {
             auto leaf = [&](uint32_t gid, const GeoV &g, float c0, float c1, float c2) {
                 if (g.op <= athr) return;
                 float dist, hx, hy, hz;
@@ -639,63 +689,222 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
         }
         if (lane == 0) {
             int32_t *ts = A.tile_stats + 4 * (size_t)(A.tile_base_launch + tile);
-            if (SRC == kFromList || !lists) ts[2] = nrounds;  // visits, leaves, hits: collect pass
-            if (!lists) {
+            if (!lists) {  // else the collect and list kernels write them
                 ts[0] = visits[0];
                 ts[1] = visits[1];
+                ts[2] = nrounds;
                 ts[3] = hsum;
             }
         }
     }
     if (!active) return;
-    const float F = T * pnc;
-    const float rem = T - F;
-    float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
-    if (ska > 0.0f) {
-        e0 = skc0 / ska;
-        e1 = skc1 / ska;
-        e2 = skc2 / ska;
+    store_pixel(A, V, pix, r, depth_on, T, pnc, skc0, skc1, skc2, ska, C0, C1, C2, D, ncomp, nhits);
+}
+
+// ---------------------------------------------------------------------------
+// Forward from the collected lists (lanes whose list is complete)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kListMark = 0x80000000u;  // list id flag: entry was composited
+constexpr int kListBatch = 8;                 // list entries loaded per step of a scan
+
+// The GRAY_KB nearest entries of a scan, sorted by (distance, id), held in
+// registers (every index is a compile-time constant after unrolling).  Empty
+// slots hold (+inf, 0xffffffff), so "x precedes the last slot" is also the
+// test for a free slot.  Insertion is a branchless merge: with lt[k] = x <
+// slot k (false...false, true...true on a sorted array), slot k becomes
+// lt[k-1] ? slot k-1 : (lt[k] ? x : slot k).
+struct RegKB {
+    float d[GRAY_KB];
+    uint32_t id[GRAY_KB];
+    uint32_t slot[GRAY_KB];
+
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int k = 0; k < GRAY_KB; ++k) {
+            d[k] = INFINITY;
+            id[k] = 0xFFFFFFFFu;
+            slot[k] = 0;
+        }
     }
-    const float cc0 = C0, cc1 = C1, cc2 = C2;
-    C0 += rem * e0;
-    C1 += rem * e1;
-    C2 += rem * e2;
-    if (depth_on) D = D / fmaxf(1.0f - T, *A.knobs.eps_fwd);
-    const float *bg = A.knobs.bg;
-    C0 += bg[0] * F;
-    C1 += bg[1] * F;
-    C2 += bg[2] * F;
-    V.out_c[3 * pix + 0] = C0;
-    V.out_c[3 * pix + 1] = C1;
-    V.out_c[3 * pix + 2] = C2;
-    if (V.out_depth) V.out_depth[pix] = D;
-    if (A.save) {
-        if (V.out_T) V.out_T[pix] = T;
-        if (V.out_F) V.out_F[pix] = F;
-        if (V.out_est) {
-            V.out_est[3 * pix + 0] = e0;
-            V.out_est[3 * pix + 1] = e1;
-            V.out_est[3 * pix + 2] = e2;
-        }
-        if (V.out_ro) {
-            V.out_ro[3 * pix + 0] = r.ox;
-            V.out_ro[3 * pix + 1] = r.oy;
-            V.out_ro[3 * pix + 2] = r.oz;
-        }
-        if (V.out_rd) {
-            V.out_rd[3 * pix + 0] = r.dx;
-            V.out_rd[3 * pix + 1] = r.dy;
-            V.out_rd[3 * pix + 2] = r.dz;
-        }
-        if (V.n_comp) V.n_comp[pix] = ncomp;
-        if (V.comp_rgb) {
-            V.comp_rgb[3 * pix + 0] = cc0;
-            V.comp_rgb[3 * pix + 1] = cc1;
-            V.comp_rgb[3 * pix + 2] = cc2;
-        }
-        if (V.stat_hits) V.stat_hits[pix] += (int)nhits;
-        if (V.stat_acc) V.stat_acc[pix] += (int)ncomp;
+    __device__ __forceinline__ bool takes(float xd, uint32_t xi) const {
+        return key_less(xd, xi, d[GRAY_KB - 1], id[GRAY_KB - 1]);
     }
+    __device__ __forceinline__ void insert(float xd, uint32_t xi, uint32_t xs) {
+        bool lt[GRAY_KB];
+#pragma unroll
+        for (int k = 0; k < GRAY_KB; ++k) lt[k] = key_less(xd, xi, d[k], id[k]);
+#pragma unroll
+        for (int k = GRAY_KB - 1; k > 0; --k) {
+            d[k] = lt[k - 1] ? d[k - 1] : (lt[k] ? xd : d[k]);
+            id[k] = lt[k - 1] ? id[k - 1] : (lt[k] ? xi : id[k]);
+            slot[k] = lt[k - 1] ? slot[k - 1] : (lt[k] ? xs : slot[k]);
+        }
+        d[0] = lt[0] ? xd : d[0];
+        id[0] = lt[0] ? xi : id[0];
+        slot[0] = lt[0] ? xs : slot[0];
+    }
+};
+
+// Same batch semantics as the re-traversal forward, with the list as the hit
+// source: a round scans the lane's list (loads issued kListBatch entries at a
+// time) into a register k-buffer keyed by (distance, id) with the list slot
+// as payload, then composites front to back and flags every composited
+// entry.  After the last round one more scan sums the unflagged entries,
+// which are exactly the skip set (hits the k-buffer never handed to
+// compositing, hits beyond zfar, hits after termination).
+__global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, const float *__restrict__ rgb0) {
+    int vi, tx, ty, tile;
+    if (!block_tile(A, vi, tx, ty, &tile)) return;
+    const DevView &V = A.views[vi];
+    const int lane = threadIdx.x;
+    const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
+    LaneRay r{};
+    bool active = false;
+    if (x < V.width && y < V.height) active = make_ray(V, x, y, r);
+    const size_t lg = (size_t)tile * GRAY_WAVE + lane;
+    const int n = A.list_n[lg];
+    active = active && n >= 0;
+    if (!__ballot(active)) return;
+
+    const size_t S = (size_t)A.list_stride;
+    const float *__restrict__ Ld = A.list_d;
+    uint32_t *__restrict__ Lid = A.list_id;
+    const float *__restrict__ La = A.list_a;
+    const float tthr = *A.knobs.t_threshold;
+    const float znear = *V.znear, zfar = *V.zfar;
+    const bool depth_on = *A.knobs.render_depth != 0;
+    const int pix = y * V.stride + x;
+    const int R = V.ray_count;
+    const float *__restrict__ rgb = rgb0 + (V.rgb - A.views[0].rgb);
+
+    RegKB kb;
+    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
+    float tmin = znear;
+    uint32_t ncomp = 0;
+    const bool traced = active && tmin < zfar;
+    bool alive = traced;
+    // round cursor, as in the re-traversal forward
+    float cur_d = znear;
+    uint32_t cur_id = 0;
+    bool strict = true;
+    int bpos = 0, batch = 0, nrounds = 0;
+
+    for (int round = 0; round < 2 * GRAY_MAX_ROUNDS * (GRAY_BATCH / GRAY_KB); ++round) {
+        if (!__ballot(alive)) break;
+        ++nrounds;
+        if (!alive) continue;
+        kb.clear();
+        int count = 0;
+        for (int i0 = 0; i0 < n; i0 += kListBatch) {
+            float d[kListBatch];
+            uint32_t id[kListBatch];
+#pragma unroll
+            for (int j = 0; j < kListBatch; ++j) {
+                d[j] = INFINITY;
+                id[j] = 0xFFFFFFFFu;
+                if (i0 + j < n) {
+                    const size_t e = (size_t)(i0 + j) * S + lg;
+                    d[j] = Ld[e];
+                    id[j] = Lid[e] & ~kListMark;
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kListBatch; ++j) {
+                const bool beyond = strict ? d[j] > cur_d : (d[j] > cur_d || (d[j] == cur_d && id[j] > cur_id));
+                if (i0 + j < n && beyond && kb.takes(d[j], id[j])) {
+                    kb.insert(d[j], id[j], (uint32_t)(i0 + j));
+                    count = count < GRAY_KB ? count + 1 : GRAY_KB;
+                }
+            }
+        }
+        bool terminated = false;
+#pragma unroll
+        for (int k = 0; k < GRAY_KB; ++k) {
+            const float dk = kb.d[k];
+            if (k >= count || terminated) continue;
+            tmin = fmaxf(dk, tmin);
+            if (!(dk < zfar)) continue;
+            const uint32_t gid = kb.id[k];
+            const size_t e = (size_t)kb.slot[k] * S + lg;
+            const float a = La[e];
+            const float c0 = rgb[3 * gid + 0], c1 = rgb[3 * gid + 1], c2 = rgb[3 * gid + 2];
+            const float w = T * a;
+            C0 += c0 * w;
+            C1 += c1 * w;
+            C2 += c2 * w;
+            if (depth_on) D += dk * w;
+            if (A.save && (int)ncomp < A.capacity) {
+                V.hit_ids[(size_t)ncomp * R + pix] = gid;
+                V.hit_T[(size_t)ncomp * R + pix] = T;
+            }
+            Lid[e] = gid | kListMark;
+            T = T * (1.0f - a);
+            ++ncomp;
+            if (T < tthr) {
+                tmin = zfar;
+                terminated = true;
+            }
+        }
+        alive = false;
+        if (!terminated) {
+            if (count < GRAY_KB) {
+                tmin = FLT_MAX;  // a partial batch ends the loop (reference: FLT_MAX slots)
+            } else {
+                bpos += GRAY_KB;
+                if (bpos >= GRAY_BATCH) {
+                    bpos = 0;
+                    strict = true;
+                    cur_d = tmin;
+                    ++batch;
+                } else {
+                    strict = false;
+                    cur_d = kb.d[GRAY_KB - 1];
+                    cur_id = kb.id[GRAY_KB - 1];
+                }
+                alive = tmin < zfar && batch < GRAY_MAX_ROUNDS;
+            }
+        }
+    }
+
+    // skip set: every entry compositing did not flag
+    float skc0 = 0.0f, skc1 = 0.0f, skc2 = 0.0f, ska = 0.0f, pnc = 1.0f;
+    const int nskip = traced ? n : 0;
+    for (int i0 = 0; i0 < nskip; i0 += kListBatch) {
+        uint32_t id[kListBatch];
+        float a[kListBatch];
+#pragma unroll
+        for (int j = 0; j < kListBatch; ++j) {
+            id[j] = kListMark;
+            a[j] = 0.0f;
+            if (i0 + j < nskip) {
+                const size_t e = (size_t)(i0 + j) * S + lg;
+                id[j] = Lid[e];
+                a[j] = La[e];
+            }
+        }
+        float c[kListBatch][3];
+#pragma unroll
+        for (int j = 0; j < kListBatch; ++j) {
+            c[j][0] = c[j][1] = c[j][2] = 0.0f;
+            if (!(id[j] & kListMark)) {
+                c[j][0] = rgb[3 * id[j] + 0];
+                c[j][1] = rgb[3 * id[j] + 1];
+                c[j][2] = rgb[3 * id[j] + 2];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kListBatch; ++j) {
+            if (id[j] & kListMark) continue;
+            skc0 += a[j] * c[j][0];
+            skc1 += a[j] * c[j][1];
+            skc2 += a[j] * c[j][2];
+            ska += a[j];
+            pnc = pnc * (1.0f - a[j]);
+        }
+    }
+    if (A.tile_stats && lane == 0) A.tile_stats[4 * (size_t)(A.tile_base_launch + tile) + 2] = nrounds;
+    if (!active) return;
+    store_pixel(A, V, pix, r, depth_on, T, pnc, skc0, skc1, skc2, ska, C0, C1, C2, D, ncomp, (uint32_t)nskip);
 }
 
 // ---------------------------------------------------------------------------
@@ -1184,10 +1393,10 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             ctx->list_lanes = lanes;
             hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, A.nodes);
             GRAY_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_trace_forward<kFromList>, grid, block, 0, s, A, A.geo, A.nodes, A.views[0].rgb);
+            hipLaunchKernelGGL(k_composite_lists, grid, block, 0, s, A, A.views[0].rgb);
             GRAY_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_trace_forward<kFromBvh>, grid, block, 0, s, A, A.geo, A.nodes, A.views[0].rgb);
+        hipLaunchKernelGGL(k_trace_forward, grid, block, 0, s, A, A.geo, A.nodes, A.views[0].rgb);
         GRAY_HIP(hipGetLastError());
     }
     return GRAY_OK;
