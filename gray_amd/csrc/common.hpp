@@ -86,14 +86,18 @@ struct TraceArgs {
     int32_t *error_flag;       // set to a GRAY_ERR_* code on device-side failure
     int32_t *tile_stats;       // optional [tiles, 4]: node visits, leaf tests, rounds, hits
     int32_t tile_base_launch, pad3;  // first tile index of this launch (views beyond 8 are launched in chunks)
-    // hit lists of the collect pass, slot-major: entry k of lane g at k*list_stride+g,
-    // g = launch tile * 64 + lane; list_n[g] = hits, -1 = incomplete (re-traverse)
-    float *list_d;
-    uint32_t *list_id;
-    float *list_a;
+    // hit lists of the collect pass, slot-major records {dist, id bits, alpha, 0}:
+    // entry k of lane g at k*list_stride+g, g = launch tile * 64 + lane;
+    // list_n[g] = hits, < 0 = incomplete (the ray is re-traversed)
+    float4 *list_rec;
     int32_t *list_n;
     int64_t list_stride;
-    int32_t list_cap, pad4;
+    int32_t list_cap;
+    int32_t budget;            // iterations of a collect walk before it hands its remaining work on
+    // continuation tasks of the collect pass, {lane g, node or leaf reference}
+    uint2 *task_in, *task_out;
+    int32_t *task_count_in, *task_count_out, *task_head;
+    int32_t task_cap, pad4;
 };
 
 // ---------------------------------------------------------------------------

@@ -38,13 +38,15 @@ struct gray_ctx {
     int64_t num_top = 0;
     int32_t *err = nullptr;         // [1] device error flag
     // hit lists of the forward collect pass (one launch of <= 8 views)
-    float *list_d = nullptr;
-    uint32_t *list_id = nullptr;
-    float *list_a = nullptr;
+    float4 *list_rec = nullptr;     // [list_entries] {dist, id bits, alpha, 0}
     int32_t *list_n = nullptr;
-    int64_t list_entries = 0;       // allocated entries of list_d/list_id/list_a
+    int64_t list_entries = 0;       // allocated records
     int64_t list_lanes_cap = 0;     // allocated entries of list_n
     int64_t list_lanes = 0;         // lanes of the last collecting launch
+    // continuation tasks of the collect pass: two ping-pong buffers + counters
+    uint2 *tasks[2] = {nullptr, nullptr};
+    int64_t task_cap = 0;
+    int32_t *task_counters = nullptr;  // [2 * kMaxTaskRounds + 2]
     // timing
     bool timing = false;
     float last_build_ms = -1.0f;

@@ -310,23 +310,227 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 // ---------------------------------------------------------------------------
 // Collect pass: every hit of every ray into its list
 // ---------------------------------------------------------------------------
-// Stack entries per lane of the collect pass (LDS).  A lane that would need
-// more falls back to re-traversal, so the bound costs speed, never results.
-constexpr int kCollectStack = 32;
-// Leaf queue per lane (LDS ring): leaves found by node visits wait here for
-// their exact test, one per iteration.
-constexpr int kLeafQueue = 8;
+// Tuning of the collect pass (compile-time; -D overrides for sweeps).
+// Stack entries per lane (LDS).  When it is full, a walk that may hand work
+// on spills the entry as a task; the last task round, which may not, has the
+// larger stack and falls back to re-traversal on overflow, so the bounds cost
+// speed, never results.
+#ifndef GRAY_COLLECT_STACK
+#define GRAY_COLLECT_STACK 16
+#endif
+#ifndef GRAY_FINAL_STACK
+#define GRAY_FINAL_STACK 64
+#endif
+// Nodes visited and leaves tested per lane and iteration.
+#ifndef GRAY_COLLECT_HEADS
+#define GRAY_COLLECT_HEADS 2
+#endif
+#ifndef GRAY_COLLECT_LEAVES
+#define GRAY_COLLECT_LEAVES 1
+#endif
+// Leaf queue per lane (LDS ring, a power of two >= 2 * heads): leaves found by
+// node visits wait here for their exact test.
+#ifndef GRAY_LEAF_QUEUE
+#define GRAY_LEAF_QUEUE 8
+#endif
+// Iterations a walk runs before it hands its remaining work on as tasks, in
+// the per-tile pass and in every task round but the last (which finishes).
+#ifndef GRAY_COLLECT_BUDGET
+#define GRAY_COLLECT_BUDGET 192
+#endif
+#ifndef GRAY_TASK_BUDGET
+#define GRAY_TASK_BUDGET 96
+#endif
+#ifndef GRAY_TASK_ROUNDS
+#define GRAY_TASK_ROUNDS 3
+#endif
+constexpr int kCollectStack = GRAY_COLLECT_STACK;
+constexpr int kFinalStack = GRAY_FINAL_STACK;
+constexpr int kCollectHeads = GRAY_COLLECT_HEADS;
+constexpr int kCollectLeaves = GRAY_COLLECT_LEAVES;
+constexpr int kLeafQueue = GRAY_LEAF_QUEUE;
+constexpr int kTaskRounds = GRAY_TASK_ROUNDS;
+constexpr int kTaskWaves = 4096;           // persistent waves of a task round
+constexpr int32_t kRayDead = -(1 << 30);   // list_n of a ray whose list or tasks overflowed
+static_assert((kLeafQueue & (kLeafQueue - 1)) == 0 && kLeafQueue >= 2 * kCollectHeads, "leaf queue size");
 
-// Each lane walks the LBVH for its own ray (the order of the walk does not
-// matter here: the list is composited in distance order afterwards).  One
-// iteration of a lane: the exact test of one queued leaf, whose record was
-// requested in the previous iteration, and the box tests of the children of
-// up to two nodes; hit leaves join the queue, hit internal children become
-// the next two nodes or go onto the stack.  Queuing keeps the expensive leaf
-// test dense across the wave (a node visit finds a leaf only about one time
-// in four); node visits pause while the queue could not take four more
-// leaves.  Every hit (distance, id, alpha) goes to the lane's list; a lane
-// whose list or stack overflows stops and is marked -1.
+__device__ __forceinline__ void put_hit(const TraceArgs &A, size_t g, int slot, float dist, uint32_t gid, float a) {
+    const size_t e = (size_t)slot * (size_t)A.list_stride + g;
+    A.list_rec[e] = make_float4(dist, __uint_as_float(gid), a, 0.0f);
+}
+
+// One lane's walk of the LBVH for its ray (the order of the walk does not
+// matter here: the list is composited in distance order afterwards), from a
+// node or leaf reference.  One iteration: the exact test of up to
+// kCollectLeaves queued leaves, whose records were requested in the previous
+// iteration, and the box tests of the children of up to kCollectHeads nodes;
+// hit leaves join the queue, hit internal children become the next nodes or
+// go onto the stack.  Queuing keeps the expensive leaf test dense across the
+// wave (a node visit finds a leaf only about one time in four).  Node visits
+// pause while the queue could not take every child.
+//
+// Every hit goes to the ray's list: at slot cnt (the per-tile pass owns the
+// ray) or at a slot taken from list_n[g] with an atomic (tasks of one ray run
+// concurrently).  After `budget` iterations (< 0: none) a lane that still has
+// work appends it to the task buffer as {g, reference} pairs and stops, so no
+// wave runs longer than the budget however many boxes its rays cross.  ok
+// turns false when the stack, the list or the task buffer overflows: the ray
+// is then re-traversed instead.  Must be called by the whole wave.
+template <bool kTask, int kStack>
+__device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord *__restrict__ geo,
+                                             const BvhNode *__restrict__ nodes, const LaneRay &r, size_t g,
+                                             uint32_t start, int budget, uint32_t *__restrict__ s_stack,
+                                             uint32_t *__restrict__ s_leaf, int lane, int &cnt, bool &ok,
+                                             int &visits, int &leaves) {
+    constexpr int NH = kCollectHeads, NL = kCollectLeaves;
+    const int cap = A.list_cap;
+    const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power;
+    // heads fill from index 0 (h[k] holds a node only when h[k-1] does), and
+    // so do the pending leaves; a pending slot is empty only when the queue is
+    uint32_t h[NH];
+    NodeV nd[NH];
+    uint32_t pl[NL];
+    GeoV pg[NL];
+#pragma unroll
+    for (int k = 0; k < NH; ++k) h[k] = GRAY_EMPTY;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) pl[l] = GRAY_EMPTY;
+    if (start != GRAY_EMPTY) {
+        if (start & GRAY_LEAF) {
+            pl[0] = start & ~GRAY_LEAF;
+            pg[0] = load_geo(geo, pl[0]);
+        } else {
+            h[0] = start;
+            nd[0] = load_node(nodes, start);
+        }
+    }
+    int sp = 0, qh = 0, qn = 0;
+    for (int it = 0;; ++it) {
+        if (!__ballot(h[0] != GRAY_EMPTY || pl[0] != GRAY_EMPTY)) break;
+        if (it == budget) {
+            // hand the remaining work on: heads, stack, pending and queued leaves
+            int m = sp + qn;
+#pragma unroll
+            for (int k = 0; k < NH; ++k) m += h[k] != GRAY_EMPTY;
+#pragma unroll
+            for (int l = 0; l < NL; ++l) m += pl[l] != GRAY_EMPTY;
+            if (m > 0) {
+                // entries past the buffer's end are dropped and the ray is
+                // re-traversed; the ones that fit are still written, so every
+                // entry below the (clamped) count is this launch's
+                const int base = atomicAdd(A.task_count_out, m);
+                if (base + m > A.task_cap) ok = false;
+                const int end = min(base + m, A.task_cap);
+                int o = base;
+                auto put = [&](uint32_t ref) {
+                    if (o < end) A.task_out[o] = make_uint2((uint32_t)g, ref);
+                    ++o;
+                };
+#pragma unroll
+                for (int k = 0; k < NH; ++k)
+                    if (h[k] != GRAY_EMPTY) put(h[k]);
+#pragma unroll
+                for (int l = 0; l < NL; ++l)
+                    if (pl[l] != GRAY_EMPTY) put(pl[l] | GRAY_LEAF);
+                for (int k = 0; k < sp; ++k) put(s_stack[k * GRAY_WAVE + lane]);
+                for (int k = 0; k < qn; ++k) put(s_leaf[((qh + k) & (kLeafQueue - 1)) * GRAY_WAVE + lane] | GRAY_LEAF);
+            }
+            break;
+        }
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            if (pl[l] == GRAY_EMPTY) continue;
+            ++leaves;
+            const GeoV &q = pg[l];
+            if (q.op <= athr) continue;
+            float dist, hx, hy, hz;
+            if (!gray_hit_test(q.m, q.mu[0], q.mu[1], q.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz))
+                continue;
+            float gv, qq;
+            const float a = gray_alpha(hx, hy, hz, q.clip, q.op, p, gv, qq);
+            if (kTask) {
+                const int slot = atomicAdd(A.list_n + g, 1);
+                if (slot >= 0 && slot < cap) put_hit(A, g, slot, dist, pl[l], a);
+                else ok = false;
+            } else {
+                if (cnt < cap) put_hit(A, g, cnt, dist, pl[l], a);
+                ++cnt;
+                if (cnt > cap) ok = false;
+            }
+        }
+        uint32_t nh[NH];
+#pragma unroll
+        for (int k = 0; k < NH; ++k) nh[k] = h[k];
+        bool moved = false;
+        if (h[0] != GRAY_EMPTY && qn <= kLeafQueue - 2 * NH) {
+            moved = true;
+            uint32_t ch[2 * NH];
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                ch[2 * k] = ch[2 * k + 1] = GRAY_EMPTY;
+                if (h[k] == GRAY_EMPTY) continue;
+                ++visits;
+                if (nd[k].left != GRAY_EMPTY && slab_test(r, nd[k].lb, 0.0f)) ch[2 * k] = nd[k].left;
+                if (nd[k].right != GRAY_EMPTY && slab_test(r, nd[k].rb, 0.0f)) ch[2 * k + 1] = nd[k].right;
+            }
+            int nfill = 0;
+#pragma unroll
+            for (int k = 0; k < NH; ++k) nh[k] = GRAY_EMPTY;
+#pragma unroll
+            for (int k = 0; k < 2 * NH; ++k) {
+                const uint32_t c = ch[k];
+                if (c == GRAY_EMPTY) continue;
+                if (c & GRAY_LEAF) {
+                    s_leaf[((qh + qn) & (kLeafQueue - 1)) * GRAY_WAVE + lane] = c & ~GRAY_LEAF;
+                    ++qn;
+                } else if (nfill < NH) {
+#pragma unroll
+                    for (int m = 0; m < NH; ++m) nh[m] = m == nfill ? c : nh[m];
+                    ++nfill;
+                } else if (sp < kStack) {
+                    s_stack[sp * GRAY_WAVE + lane] = c;
+                    ++sp;
+                } else if (budget >= 0) {
+                    const int o = atomicAdd(A.task_count_out, 1);
+                    if (o < A.task_cap) A.task_out[o] = make_uint2((uint32_t)g, c);
+                    else ok = false;
+                } else {
+                    ok = false;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < NH; ++m) {
+                if (m >= nfill && sp > 0) {
+                    --sp;
+                    nh[m] = s_stack[sp * GRAY_WAVE + lane];
+                }
+            }
+        }
+        if (!ok) {
+#pragma unroll
+            for (int k = 0; k < NH; ++k) nh[k] = GRAY_EMPTY;
+            sp = qn = 0;
+        }
+        // requests: the next leaf records first, then the next nodes
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            pl[l] = GRAY_EMPTY;
+            if (qn > 0) {
+                pl[l] = s_leaf[qh * GRAY_WAVE + lane];
+                qh = (qh + 1) & (kLeafQueue - 1);
+                --qn;
+                pg[l] = load_geo(geo, pl[l]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            h[k] = nh[k];
+            if (moved && h[k] != GRAY_EMPTY) nd[k] = load_node(nodes, h[k]);
+        }
+    }
+}
+
+// Per-tile collect pass: every ray of the tile walks from the root.
 __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRecord *__restrict__ geo,
                                                        const BvhNode *__restrict__ nodes) {
     __shared__ uint32_t s_stack[kCollectStack * GRAY_WAVE];
@@ -340,103 +544,10 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
     bool active = false;
     if (x < V.width && y < V.height) active = make_ray(V, x, y, r);
     const size_t g = (size_t)tile * GRAY_WAVE + lane;
-    const size_t S = (size_t)A.list_stride;
-    const int cap = A.list_cap;
-    const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power;
-    float *__restrict__ out_d = A.list_d;
-    uint32_t *__restrict__ out_id = A.list_id;
-    float *__restrict__ out_a = A.list_a;
-
     int cnt = 0, visits = 0, leaves = 0;
     bool ok = true;
-    uint32_t h0 = (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY, h1 = GRAY_EMPTY;
-    NodeV n0, n1;
-    if (h0 != GRAY_EMPTY) n0 = load_node(nodes, h0);
-    uint32_t pl = GRAY_EMPTY;  // leaf whose record pg is in flight; GRAY_EMPTY implies an empty queue
-    GeoV pg;
-    int sp = 0, qh = 0, qn = 0;
-    for (;;) {
-        // invariants: h1 holds a node only when h0 does; pl is empty only when the queue is
-        if (!__ballot(h0 != GRAY_EMPTY || pl != GRAY_EMPTY)) break;
-        if (pl != GRAY_EMPTY) {
-            ++leaves;
-            if (pg.op > athr) {
-                float dist, hx, hy, hz;
-                if (gray_hit_test(pg.m, pg.mu[0], pg.mu[1], pg.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx,
-                                  hy, hz)) {
-                    float gv, qq;
-                    const float a = gray_alpha(hx, hy, hz, pg.clip, pg.op, p, gv, qq);
-                    if (cnt < cap) {
-                        const size_t e = (size_t)cnt * S + g;
-                        out_d[e] = dist;
-                        out_id[e] = pl;
-                        out_a[e] = a;
-                    }
-                    ++cnt;
-                }
-            }
-        }
-        uint32_t nh0 = h0, nh1 = h1;
-        bool moved = false;
-        if (h0 != GRAY_EMPTY && qn <= kLeafQueue - 4) {
-            moved = true;
-            uint32_t ch[4] = {GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY};
-            ++visits;
-            if (n0.left != GRAY_EMPTY && slab_test(r, n0.lb, 0.0f)) ch[0] = n0.left;
-            if (n0.right != GRAY_EMPTY && slab_test(r, n0.rb, 0.0f)) ch[1] = n0.right;
-            if (h1 != GRAY_EMPTY) {
-                ++visits;
-                if (n1.left != GRAY_EMPTY && slab_test(r, n1.lb, 0.0f)) ch[2] = n1.left;
-                if (n1.right != GRAY_EMPTY && slab_test(r, n1.rb, 0.0f)) ch[3] = n1.right;
-            }
-            nh0 = nh1 = GRAY_EMPTY;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint32_t c = ch[k];
-                if (c == GRAY_EMPTY) continue;
-                if (c & GRAY_LEAF) {
-                    s_leaf[((qh + qn) & (kLeafQueue - 1)) * GRAY_WAVE + lane] = c & ~GRAY_LEAF;
-                    ++qn;
-                } else if (nh0 == GRAY_EMPTY) {
-                    nh0 = c;
-                } else if (nh1 == GRAY_EMPTY) {
-                    nh1 = c;
-                } else if (sp < kCollectStack) {
-                    s_stack[sp * GRAY_WAVE + lane] = c;
-                    ++sp;
-                } else {
-                    ok = false;
-                }
-            }
-            if (nh0 == GRAY_EMPTY && sp > 0) {
-                --sp;
-                nh0 = s_stack[sp * GRAY_WAVE + lane];
-            }
-            if (nh1 == GRAY_EMPTY && sp > 0) {
-                --sp;
-                nh1 = s_stack[sp * GRAY_WAVE + lane];
-            }
-        }
-        if (cnt > cap) ok = false;
-        if (!ok) {
-            nh0 = nh1 = GRAY_EMPTY;
-            sp = qn = 0;
-        }
-        // requests: the next leaf record first, then the next nodes
-        pl = GRAY_EMPTY;
-        if (qn > 0) {
-            pl = s_leaf[qh * GRAY_WAVE + lane];
-            qh = (qh + 1) & (kLeafQueue - 1);
-            --qn;
-            pg = load_geo(geo, pl);
-        }
-        h0 = nh0;
-        h1 = nh1;
-        if (moved) {
-            if (h0 != GRAY_EMPTY) n0 = load_node(nodes, h0);
-            if (h1 != GRAY_EMPTY) n1 = load_node(nodes, h1);
-        }
-    }
+    collect_walk<false, kCollectStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
+                                       A.budget, s_stack, s_leaf, lane, cnt, ok, visits, leaves);
     A.list_n[g] = ok ? cnt : -1;
     if (A.tile_stats) {
         int hsum = ok ? cnt : 0;
@@ -453,6 +564,62 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
             ts[3] = hsum;
         }
     }
+}
+
+// Tile, view and pixel of launch lane g (the inverse of the tile numbering).
+__device__ __forceinline__ const DevView &lane_pixel(const TraceArgs &A, uint32_t g, int &x, int &y) {
+    const int tile = (int)(g / GRAY_WAVE), lane = (int)(g % GRAY_WAVE);
+    int v = 0;
+#pragma unroll
+    for (int k = 1; k < GRAY_MAX_VIEWS; ++k)
+        if (k < A.num_views && tile >= A.views[k].tile_base) v = k;
+    const DevView &V = A.views[v];
+    const int lt = tile - V.tile_base;
+    x = (lt % V.tiles_x) * GRAY_TILE + (lane & 7);
+    y = (lt / V.tiles_x) * GRAY_TILE + (lane >> 3);
+    return V;
+}
+
+// A task round: persistent waves take the round's tasks 64 at a time (one per
+// lane) and walk each task's subtree for its ray.  The task count was fixed by
+// the previous kernel, so every wave leaves the loop once the shared head
+// passes it.
+template <int kStack>
+__global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const GeoRecord *__restrict__ geo,
+                                                             const BvhNode *__restrict__ nodes) {
+    __shared__ uint32_t s_stack[kStack * GRAY_WAVE];
+    __shared__ uint32_t s_leaf[kLeafQueue * GRAY_WAVE];
+    const int lane = threadIdx.x;
+    const int total = min(*A.task_count_in, A.task_cap);
+    for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(A.task_head, GRAY_WAVE);
+        base = __shfl(base, 0);
+        if (base >= total) break;
+        const int t = base + lane;
+        uint32_t start = GRAY_EMPTY;
+        uint32_t g = 0;
+        LaneRay r{};
+        if (t < total) {
+            const uint2 task = A.task_in[t];
+            if (task.x < (uint64_t)A.list_stride) {
+                g = task.x;
+                int x, y;
+                const DevView &V = lane_pixel(A, g, x, y);
+                if (x < V.width && y < V.height && A.list_n[g] >= 0 && make_ray(V, x, y, r)) start = task.y;
+            }
+        }
+        int cnt = 0, visits = 0, leaves = 0;
+        bool ok = true;
+        collect_walk<true, kStack>(A, geo, nodes, r, g, start, A.budget, s_stack, s_leaf, lane, cnt, ok, visits,
+                                   leaves);
+        if (!ok && start != GRAY_EMPTY) atomicExch(A.list_n + g, kRayDead);
+    }
+}
+
+__global__ void k_zero_i32(int32_t *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 // Pixel outputs of the forward (reference forward_pass.cu:232-268): colour
@@ -767,9 +934,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
     if (!__ballot(active)) return;
 
     const size_t S = (size_t)A.list_stride;
-    const float *__restrict__ Ld = A.list_d;
-    uint32_t *__restrict__ Lid = A.list_id;
-    const float *__restrict__ La = A.list_a;
+    float4 *__restrict__ L = A.list_rec;
     const float tthr = *A.knobs.t_threshold;
     const float znear = *V.znear, zfar = *V.zfar;
     const bool depth_on = *A.knobs.render_depth != 0;
@@ -803,9 +968,9 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
                 d[j] = INFINITY;
                 id[j] = 0xFFFFFFFFu;
                 if (i0 + j < n) {
-                    const size_t e = (size_t)(i0 + j) * S + lg;
-                    d[j] = Ld[e];
-                    id[j] = Lid[e] & ~kListMark;
+                    const float4 rec = L[(size_t)(i0 + j) * S + lg];
+                    d[j] = rec.x;
+                    id[j] = __float_as_uint(rec.y) & ~kListMark;
                 }
             }
 #pragma unroll
@@ -826,7 +991,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
             if (!(dk < zfar)) continue;
             const uint32_t gid = kb.id[k];
             const size_t e = (size_t)kb.slot[k] * S + lg;
-            const float a = La[e];
+            const float a = L[e].z;
             const float c0 = rgb[3 * gid + 0], c1 = rgb[3 * gid + 1], c2 = rgb[3 * gid + 2];
             const float w = T * a;
             C0 += c0 * w;
@@ -837,7 +1002,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
                 V.hit_ids[(size_t)ncomp * R + pix] = gid;
                 V.hit_T[(size_t)ncomp * R + pix] = T;
             }
-            Lid[e] = gid | kListMark;
+            reinterpret_cast<uint32_t *>(L + e)[1] = gid | kListMark;
             T = T * (1.0f - a);
             ++ncomp;
             if (T < tthr) {
@@ -877,9 +1042,9 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
             id[j] = kListMark;
             a[j] = 0.0f;
             if (i0 + j < nskip) {
-                const size_t e = (size_t)(i0 + j) * S + lg;
-                id[j] = Lid[e];
-                a[j] = La[e];
+                const float4 rec = L[(size_t)(i0 + j) * S + lg];
+                id[j] = __float_as_uint(rec.y);
+                a[j] = rec.z;
             }
         }
         float c[kListBatch][3];
@@ -1308,36 +1473,53 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
     return GRAY_OK;
 }
 
-// Grows the collect-pass lists to lanes * cap entries (never shrinks).
+// Grows the collect-pass lists to lanes * cap records and the task buffers
+// to kTaskPerLane entries per lane (never shrinks).
+constexpr int64_t kTaskPerLane = 16;
+
 int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
     if (cap > (1 << 16)) {
         gray_set_error(GRAY_ERR_INVALID, "render: list_capacity %d exceeds 65536", cap);
         return GRAY_ERR_INVALID;
     }
-    const int64_t entries = lanes * (int64_t)cap;
+    if (lanes * kTaskPerLane > INT32_MAX) {
+        gray_set_error(GRAY_ERR_INVALID, "render: %lld rays in one launch exceed the task buffer index range",
+                       (long long)lanes);
+        return GRAY_ERR_INVALID;
+    }
+    auto fail = [&](const char *what, int64_t bytes) {
+        (void)hipGetLastError();
+        gray_set_error(GRAY_ERR_NOMEM, "render: cannot allocate %s (%lld bytes)", what, (long long)bytes);
+        return GRAY_ERR_NOMEM;
+    };
+    if (!ctx->task_counters) {
+        if (hipMalloc(&ctx->task_counters, sizeof(int32_t) * (2 * kTaskRounds + 2)) != hipSuccess)
+            return fail("task counters", sizeof(int32_t) * (2 * kTaskRounds + 2));
+    }
     if (lanes > ctx->list_lanes_cap) {
-        if (ctx->list_n) GRAY_HIP(hipFree(ctx->list_n));
+        GRAY_HIP(hipFree(ctx->list_n));
+        for (uint2 *&t : ctx->tasks) {
+            GRAY_HIP(hipFree(t));
+            t = nullptr;
+        }
         ctx->list_n = nullptr;
         ctx->list_lanes_cap = 0;
-        GRAY_HIP(hipMalloc(&ctx->list_n, sizeof(int32_t) * lanes));
+        ctx->task_cap = 0;
+        if (hipMalloc(&ctx->list_n, sizeof(int32_t) * lanes) != hipSuccess)
+            return fail("list counts", sizeof(int32_t) * lanes);
+        for (uint2 *&t : ctx->tasks)
+            if (hipMalloc(&t, sizeof(uint2) * lanes * kTaskPerLane) != hipSuccess)
+                return fail("task buffers", sizeof(uint2) * lanes * kTaskPerLane);
         ctx->list_lanes_cap = lanes;
+        ctx->task_cap = lanes * kTaskPerLane;
     }
+    const int64_t entries = lanes * (int64_t)cap;
     if (entries > ctx->list_entries) {
-        void *bufs[3] = {ctx->list_d, ctx->list_id, ctx->list_a};
-        for (void *b : bufs)
-            if (b) GRAY_HIP(hipFree(b));
-        ctx->list_d = nullptr;
-        ctx->list_id = nullptr;
-        ctx->list_a = nullptr;
+        GRAY_HIP(hipFree(ctx->list_rec));
+        ctx->list_rec = nullptr;
         ctx->list_entries = 0;
-        if (hipMalloc(&ctx->list_d, sizeof(float) * entries) != hipSuccess ||
-            hipMalloc(&ctx->list_id, sizeof(uint32_t) * entries) != hipSuccess ||
-            hipMalloc(&ctx->list_a, sizeof(float) * entries) != hipSuccess) {
-            (void)hipGetLastError();
-            gray_set_error(GRAY_ERR_NOMEM, "render: cannot allocate hit lists (%lld entries of 12 bytes)",
-                           (long long)entries);
-            return GRAY_ERR_NOMEM;
-        }
+        if (hipMalloc(&ctx->list_rec, sizeof(float4) * entries) != hipSuccess)
+            return fail("hit lists", (int64_t)sizeof(float4) * entries);
         ctx->list_entries = entries;
     }
     return GRAY_OK;
@@ -1384,15 +1566,37 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
         if (args->list_capacity > 0 && A.use_bvh && A.num_gaussians > 0) {
             const int64_t lanes = (int64_t)A.total_tiles * GRAY_WAVE;
             if ((rc = reserve_lists(ctx, lanes, args->list_capacity))) return rc;
-            A.list_d = ctx->list_d;
-            A.list_id = ctx->list_id;
-            A.list_a = ctx->list_a;
+            A.list_rec = ctx->list_rec;
             A.list_n = ctx->list_n;
             A.list_stride = lanes;
             A.list_cap = args->list_capacity;
+            A.task_cap = (int32_t)ctx->task_cap;
             ctx->list_lanes = lanes;
+            // counters: count[r] tasks handed to round r, head[r] its take position
+            int32_t *count = ctx->task_counters, *head = ctx->task_counters + kTaskRounds + 1;
+            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, ctx->task_counters, 2 * kTaskRounds + 2);
+            GRAY_HIP(hipGetLastError());
+            A.task_out = ctx->tasks[0];
+            A.task_count_out = count;
+            A.budget = args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
             hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, A.nodes);
             GRAY_HIP(hipGetLastError());
+            for (int round = 0; round < kTaskRounds; ++round) {
+                A.task_in = ctx->tasks[round & 1];
+                A.task_count_in = count + round;
+                A.task_head = head + round;
+                A.task_out = ctx->tasks[(round + 1) & 1];
+                A.task_count_out = count + round + 1;
+                A.budget = round + 1 < kTaskRounds ? (args->collect_budget > 0 ? args->collect_budget : GRAY_TASK_BUDGET)
+                                                   : -1;
+                if (A.budget >= 0)
+                    hipLaunchKernelGGL(k_collect_tasks<kCollectStack>, dim3(kTaskWaves), block, 0, s, A, A.geo,
+                                       A.nodes);
+                else
+                    hipLaunchKernelGGL(k_collect_tasks<kFinalStack>, dim3(kTaskWaves), block, 0, s, A, A.geo,
+                                       A.nodes);
+                GRAY_HIP(hipGetLastError());
+            }
             hipLaunchKernelGGL(k_composite_lists, grid, block, 0, s, A, A.views[0].rgb);
             GRAY_HIP(hipGetLastError());
         }

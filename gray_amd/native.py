@@ -21,7 +21,7 @@ import torch
 from . import _lib as L
 
 F32 = torch.float32
-# hits per ray kept by the forward's collect pass (12 bytes each): 512 covers
+# hits per ray kept by the forward's collect pass (16 bytes each): 512 covers
 # the grazing rays of dense surface scenes; longer lists re-traverse
 DEFAULT_LIST_CAPACITY = 512
 
@@ -318,6 +318,7 @@ class NativeRaytracer:
         # hits collected per ray before compositing (0: composite while
         # traversing); rays with more hits are re-traversed per batch
         self.list_capacity = DEFAULT_LIST_CAPACITY
+        self.collect_budget = 0  # walk iterations before work is split into tasks (0: built-in)
         self.tile_stats = None  # optional int32 [tiles, 4] traversal statistics of the forward
         self.timer = _NoTimer()
         self._steps = 0
@@ -376,6 +377,7 @@ class NativeRaytracer:
         a.use_bvh = 1 if self.use_bvh else 0
         a.tile_stats = self.tile_stats.data_ptr() if self.tile_stats is not None else None
         a.list_capacity = int(self.list_capacity)
+        a.collect_budget = int(self.collect_budget)
         return a
 
     def _require_bvh(self) -> None:

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 2
+#define GRAY_ABI_VERSION 3
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -123,7 +123,11 @@ typedef struct {
     int32_t list_capacity;      /* hits collected per ray by the forward's traversal pass before
                                    compositing (0 = composite during traversal, re-traversing per batch).
                                    Rays with more hits fall back to re-traversal.  The context keeps
-                                   list_capacity * 12 bytes per ray of one launch (<= 8 views). */
+                                   list_capacity * 16 bytes per ray of one launch (<= 8 views),
+                                   plus 128 bytes per ray of continuation tasks. */
+    int32_t collect_budget;     /* iterations a ray's walk runs before its remaining subtrees become tasks
+                                   that other lanes take (load balance of rays that cross many boxes);
+                                   <= 0 = the built-in default */
 } gray_render_args;
 
 /* Gradient targets of the backward (reference gaussians.h:34-62). */
@@ -269,7 +273,7 @@ int gray_debug_bvh(gray_ctx *ctx, float *nodes, uint32_t *sorted_ids, uint32_t *
 /* No reference counterpart (diagnostic of the collect pass).  Copies the hit
  * counts per ray of the last forward launch that collected lists (device
  * buffer, int32 [tiles*64], tile-major in the launch's tile order, lane =
- * 8*(y%8) + x%8; -1 = the ray fell back to re-traversal).  *lanes receives
+ * 8*(y%8) + x%8; negative = the ray fell back to re-traversal).  *lanes receives
  * the number of entries; counts may be NULL to query it. */
 int gray_debug_list_counts(gray_ctx *ctx, int32_t *counts, int64_t max_lanes, int64_t *lanes, void *stream);
 

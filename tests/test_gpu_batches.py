@@ -31,7 +31,9 @@ def axis_scene(dup_at=None):
 def run(mean, q, s, logit, colors, tthr, capacity, list_capacity=512):
     W = H = 8
     nr = NativeRaytracer(W, H, N_AXIS, 0, 1, 1, hit_capacity=capacity)
-    nr.list_capacity = list_capacity
+    # a negative capacity means: lists of that size, walks split into tasks every iteration
+    nr.list_capacity = abs(list_capacity)
+    nr.collect_budget = 1 if list_capacity < 0 else 0
     g = nr.get_gaussians()
     cfg = nr.get_config()
     cfg.t_threshold.fill_(tthr)
@@ -57,7 +59,7 @@ def run(mean, q, s, logit, colors, tthr, capacity, list_capacity=512):
     return nr
 
 
-@pytest.mark.parametrize("list_capacity", [512, 0, 64], ids=["lists", "traverse", "overflow"])
+@pytest.mark.parametrize("list_capacity", [512, 0, 64, -512], ids=["lists", "traverse", "overflow", "tasks"])
 @pytest.mark.parametrize("dup_at,tthr,expect_drop", [(None, 0.0, 0), (None, 0.05, 0), (15, 0.0, 0), (31, 0.0, 1),
                                                      (47, 0.0, 0), (63, 0.0, 1)])
 def test_batches_and_ties(dup_at, tthr, expect_drop, list_capacity):

@@ -16,14 +16,18 @@ pytestmark = pytest.mark.gpu
 
 
 # hit sources of the forward: collected lists (default), composite while
-# traversing (0), and lists so short that most rays of a tile fall back to
-# re-traversal while their neighbours composite from lists (3)
-@pytest.fixture(scope="module", params=[None, 0, 3], ids=["lists", "traverse", "mixed"])
+# traversing (0), lists so short that most rays of a tile fall back to
+# re-traversal while their neighbours composite from lists (3), and lists
+# whose walks stop every 2 iterations so that every ray's remaining subtrees
+# are finished by the task rounds
+@pytest.fixture(scope="module", params=[(None, 0), (0, 0), (3, 0), (None, 2)],
+                ids=["lists", "traverse", "mixed", "tasks"])
 def tiny(request):
     torch.manual_seed(0)
     scene = scenes.tiny_scene(1024, seed=0)
     cam = scenes.tiny_camera(64, 64)
-    nr = setup_native(scene, cam, list_capacity=request.param)
+    cap, budget = request.param
+    nr = setup_native(scene, cam, list_capacity=cap, collect_budget=budget)
     torch.set_grad_enabled(True)
     nr.forward_pass()
     torch.cuda.synchronize()
