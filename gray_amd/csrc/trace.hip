@@ -1083,8 +1083,75 @@ struct PixelState {
     float pixels;          // width*height of the launch (pruning weight normaliser)
 };
 
+// Gradient terms of one hit, in the order of kGradTerms: colour 3, rotation 4,
+// scale 3, mean 3, opacity 1, pruning weight 1.
+constexpr int kGradTerms = 15;
+
+// Adds the gradient terms of one Gaussian to the reference gradient tensors.
+__device__ __forceinline__ void grad_to_global(const TraceArgs &A, const DevView &V, uint32_t gid, const float *t) {
+    atomicAdd(&V.grad_rgb[3 * gid + 0], t[0]);
+    atomicAdd(&V.grad_rgb[3 * gid + 1], t[1]);
+    atomicAdd(&V.grad_rgb[3 * gid + 2], t[2]);
+    atomicAdd(&A.g_rot[4 * gid + 0], t[3]);
+    atomicAdd(&A.g_rot[4 * gid + 1], t[4]);
+    atomicAdd(&A.g_rot[4 * gid + 2], t[5]);
+    atomicAdd(&A.g_rot[4 * gid + 3], t[6]);
+    atomicAdd(&A.g_scale[3 * gid + 0], t[7]);
+    atomicAdd(&A.g_scale[3 * gid + 1], t[8]);
+    atomicAdd(&A.g_scale[3 * gid + 2], t[9]);
+    atomicAdd(&A.g_mean[3 * gid + 0], t[10]);
+    atomicAdd(&A.g_mean[3 * gid + 1], t[11]);
+    atomicAdd(&A.g_mean[3 * gid + 2], t[12]);
+    atomicAdd(&A.g_op[gid], t[13]);
+    if (A.g_prune) atomicAdd(&A.g_prune[gid], t[14]);
+    if (A.visible) A.visible[gid] = 1;
+}
+
+// Per-wave gradient accumulator in LDS: the rays of a tile composite largely
+// the same Gaussians, so their terms are summed here (ds_add_f32) and each
+// Gaussian reaches the global tensors once per tile.  Open addressing with a
+// bounded probe; a Gaussian that finds no slot goes to the global tensors
+// directly.  Terms are stored [term][slot].
+constexpr int kGradSlots = 128;
+
+struct GradTable {
+    uint32_t *key;  // [kGradSlots], GRAY_EMPTY = free
+    float *val;     // [kGradTerms][kGradSlots]
+
+    __device__ __forceinline__ void clear(int lane) {
+        for (int i = lane; i < kGradSlots; i += GRAY_WAVE) key[i] = GRAY_EMPTY;
+        for (int i = lane; i < kGradSlots * kGradTerms; i += GRAY_WAVE) val[i] = 0.0f;
+    }
+    __device__ __forceinline__ void add(const TraceArgs &A, const DevView &V, uint32_t gid, const float *t) {
+        uint32_t slot = (gid * 2654435761u) >> 25;  // 7 bits
+        for (int probe = 0; probe < 8; ++probe) {
+            const uint32_t k = atomicCAS(&key[slot], GRAY_EMPTY, gid);
+            if (k == GRAY_EMPTY || k == gid) {
+#pragma unroll
+                for (int j = 0; j < kGradTerms; ++j) atomicAdd(&val[j * kGradSlots + slot], t[j]);
+                return;
+            }
+            slot = (slot + 1) & (kGradSlots - 1);
+        }
+        grad_to_global(A, V, gid, t);
+    }
+    __device__ __forceinline__ void flush(const TraceArgs &A, const DevView &V, int lane) const {
+        for (int i = lane; i < kGradSlots; i += GRAY_WAVE) {
+            const uint32_t gid = key[i];
+            if (gid == GRAY_EMPTY) continue;
+            float t[kGradTerms];
+#pragma unroll
+            for (int j = 0; j < kGradTerms; ++j) t[j] = val[j * kGradSlots + i];
+            grad_to_global(A, V, gid, t);
+        }
+    }
+};
+
+// Gradient of one composited hit; the terms go to sink(gid, terms).
+template <typename Sink>
 __device__ __forceinline__ void backward_hit(const TraceArgs &A, const DevView &V, const LaneRay &r, float p,
-                                             float eps_sg, const PixelState &ps, uint32_t gid, float Ti, float &gT) {
+                                             float eps_sg, const PixelState &ps, uint32_t gid, float Ti, float &gT,
+                                             Sink &&sink) {
 #pragma clang fp contract(off)
     const GeoRecord rec = A.geo[gid];
     float dist, hx, hy, hz;
@@ -1094,10 +1161,11 @@ __device__ __forceinline__ void backward_hit(const TraceArgs &A, const DevView &
     const float a = gray_alpha(hx, hy, hz, rec.clip, rec.op, p, gv, q);
     const float c0 = V.rgb[3 * gid + 0], c1 = V.rgb[3 * gid + 1], c2 = V.rgb[3 * gid + 2];
     const float w = Ti * a;
+    float t[kGradTerms];
     // colour gradient
-    atomicAdd(&V.grad_rgb[3 * gid + 0], ps.g0 * w);
-    atomicAdd(&V.grad_rgb[3 * gid + 1], ps.g1 * w);
-    atomicAdd(&V.grad_rgb[3 * gid + 2], ps.g2 * w);
+    t[0] = ps.g0 * w;
+    t[1] = ps.g1 * w;
+    t[2] = ps.g2 * w;
     // alpha gradient
     const float gdot = (ps.g0 * c0 + ps.g1 * c1) + ps.g2 * c2;
     const float oma = 1.0f - a;
@@ -1158,19 +1226,19 @@ __device__ __forceinline__ void backward_hit(const TraceArgs &A, const DevView &
     const float q2 = (qdot * -qy) / len3 + gy / qlen;
     const float q3 = (qdot * -qz) / len3 + gz / qlen;
 
-    atomicAdd(&A.g_rot[4 * gid + 0], q0);
-    atomicAdd(&A.g_rot[4 * gid + 1], q1);
-    atomicAdd(&A.g_rot[4 * gid + 2], q2);
-    atomicAdd(&A.g_rot[4 * gid + 3], q3);
-    atomicAdd(&A.g_scale[3 * gid + 0], gs0);
-    atomicAdd(&A.g_scale[3 * gid + 1], gs1);
-    atomicAdd(&A.g_scale[3 * gid + 2], gs2);
-    atomicAdd(&A.g_mean[3 * gid + 0], -gxw0);
-    atomicAdd(&A.g_mean[3 * gid + 1], -gxw1);
-    atomicAdd(&A.g_mean[3 * gid + 2], -gxw2);
-    atomicAdd(&A.g_op[gid], gop);
-    if (A.g_prune) atomicAdd(&A.g_prune[gid], w / ps.pixels);
-    if (A.visible) A.visible[gid] = 1;
+    t[3] = q0;
+    t[4] = q1;
+    t[5] = q2;
+    t[6] = q3;
+    t[7] = gs0;
+    t[8] = gs1;
+    t[9] = gs2;
+    t[10] = -gxw0;
+    t[11] = -gxw1;
+    t[12] = -gxw2;
+    t[13] = gop;
+    t[14] = w / ps.pixels;
+    sink(gid, t);
 }
 
 __device__ __forceinline__ bool load_pixel_state(const TraceArgs &A, const DevView &V, int pix, LaneRay &r,
@@ -1201,26 +1269,33 @@ __device__ __forceinline__ bool load_pixel_state(const TraceArgs &A, const DevVi
 
 // Rays whose composited list fits the store: replay it back to front.
 __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored(TraceArgs A) {
+    __shared__ uint32_t s_key[kGradSlots];
+    __shared__ float s_val[kGradSlots * kGradTerms];
     int vi, tx, ty;
     if (!block_tile(A, vi, tx, ty)) return;
     const DevView &V = A.views[vi];
     const int lane = threadIdx.x;
+    GradTable tab{s_key, s_val};
+    tab.clear(lane);
+    __syncthreads();
     const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
-    if (x >= V.width || y >= V.height) return;
     const int pix = y * V.stride + x;
-    const int n = (int)V.n_comp[pix];
-    if (n == 0 || n > A.capacity) return;
+    const int n = (x < V.width && y < V.height) ? (int)V.n_comp[pix] : 0;
     LaneRay r;
     PixelState ps;
-    if (!load_pixel_state(A, V, pix, r, ps)) return;
-    const float p = *A.knobs.exp_power, eps_sg = *A.knobs.eps_scale_grad;
-    const int R = V.ray_count;
-    float gT = 0.0f;
-    for (int k = n - 1; k >= 0; --k) {
-        const uint32_t gid = V.hit_ids[(size_t)k * R + pix];
-        const float Ti = V.hit_T[(size_t)k * R + pix];
-        backward_hit(A, V, r, p, eps_sg, ps, gid, Ti, gT);
+    if (n > 0 && n <= A.capacity && load_pixel_state(A, V, pix, r, ps)) {
+        const float p = *A.knobs.exp_power, eps_sg = *A.knobs.eps_scale_grad;
+        const int R = V.ray_count;
+        float gT = 0.0f;
+        auto sink = [&](uint32_t gid, const float *t) { tab.add(A, V, gid, t); };
+        for (int k = n - 1; k >= 0; --k) {
+            const uint32_t gid = V.hit_ids[(size_t)k * R + pix];
+            const float Ti = V.hit_T[(size_t)k * R + pix];
+            backward_hit(A, V, r, p, eps_sg, ps, gid, Ti, gT, sink);
+        }
     }
+    __syncthreads();
+    tab.flush(A, V, lane);
 }
 
 // Rays whose list exceeded the store (or capacity 0): regenerate each round by
@@ -1321,7 +1396,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
             for (int k = used - 1; k >= 0; --k) {
                 const uint32_t gid = kb.gid(k);
                 if (gid == GRAY_EMPTY) continue;
-                backward_hit(A, V, r, p, eps_sg, ps, gid, kb.alpha(k), gT);
+                backward_hit(A, V, r, p, eps_sg, ps, gid, kb.alpha(k), gT,
+                             [&](uint32_t g, const float *t) { grad_to_global(A, V, g, t); });
             }
             alive = false;
             if (!terminated && !last && count == GRAY_KB) {
