@@ -5,17 +5,23 @@ Gaussians on 20 random surfaces (log-scale N(-4.5, 0.4^2)), one 980x545
 pinhole camera (fov 60 deg) on a ring of radius 6; per step: forward render,
 L1 loss against a random target, backward, fused Adam and BVH refit.
 Prints one JSON line (metric/value/unit/...) with the dominant kernel's HBM
-roofline and a bounded CPU-oracle baseline timed on the host cores.
+roofline (HIP-event launch time, HBM traffic from rocprofv3 counters) and a
+bounded CPU-oracle baseline timed on the host cores.
 
-  python bench.py [--steps K] [--warmup W] [--workload train|render] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-traffic]
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import math
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -24,22 +30,33 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "train-step ms & rays/s at 1M Gaussians; p50/p99 frame latency vs plain launch"
-
-# algorithmic (compulsory) bytes per launch of each native stage:
-# per Gaussian / per ray terms, see DESIGN.md "Measurement"
-BYTES_PER_GAUSSIAN = {
-    "trace_forward": 76,     # geometry record 64 B + colour 12 B
-    "trace_backward": 216,   # geometry 64 + L2W 48 + rgb 12 + live params 32 + grads out 60
-    "sh_forward": 216,       # mean 12 + SH 192 in, colour 12 out
-    "sh_backward": 433,      # mean, SH rest, colour, colour grad, visibility in; SH grads out, mean grad r/w
-    "adam": 1905,            # 59 floats x (p,g,m,v in + p,m,v,g out) + channel-grad zeroing + pruning counter
-    "bvh_refit": 300,        # snapshot 44 in / 136 out + refit boxes, scans and node writes
-}
-BYTES_PER_RAY = {"trace_forward": 72, "trace_backward": 60}
+DOMINANT_KERNEL = "k_collect"
+# algorithmic bytes of the collect kernel per Gaussian: the parameters its
+# exact hit tests depend on (mean 12, rotation 16, scale 12, opacity 4), read
+# once; hit lists, BVH nodes and the snapshot are implementation traffic
+COLLECT_BYTES_PER_GAUSSIAN = 44
+# algorithmic bytes of a whole training step (params, grads, Adam state and
+# images, each moved once): 2640 per Gaussian + 60 per ray
+STEP_BYTES_PER_GAUSSIAN = 2640
+STEP_BYTES_PER_RAY = 60
+# calibration kernel of the traffic counters: neg over 128 Mi floats reads and
+# writes 512 MiB each
+CAL_FLOATS = 128 * 1024 * 1024
+CAL_KIB = CAL_FLOATS * 4 // 1024
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
+
+
+def train_scene(n: int):
+    from gray_amd import RaytracerConfig, scenes
+
+    W, H = 980, 545
+    scene = scenes.surface_scene(n, 20, -4.5, 0.4, seed=0)
+    cam = scenes.ring_camera(6.0, 0.0, W, H, 60.0)
+    cfg = RaytracerConfig(lr_mean_init=1.6e-4, lr_scale_init=5e-3, lr_sh_dc_init=2.5e-3, lr_sh_rest=2.5e-3)
+    return scene, cam, cfg
 
 
 def setup_raytracer(scene, cam, cfg):
@@ -59,6 +76,18 @@ def setup_raytracer(scene, cam, cfg):
         g.current_sh_degree.fill_(3)
     rt.cuda_module.rebuild_bvh()
     return rt
+
+
+def make_step(rt, cam, target):
+    import torch
+
+    def step():
+        render = rt(cam)
+        loss = torch.nn.functional.l1_loss(render, target)
+        rt.backward(loss)
+        rt.step()
+
+    return step
 
 
 def cpu_baseline(scene, cam, cfg, sample_rays: int, seed: int = 0):
@@ -92,32 +121,108 @@ def cpu_baseline(scene, cam, cfg, sample_rays: int, seed: int = 0):
         "cores": O.num_threads(),
         "kind": "port",
         "sample": f"{rays} seeded pixels of the {W}x{H} frame, forward+backward against all {scene.count} "
-                  f"Gaussians (brute force, C oracle); per-Gaussian prep {t1 - t0:.2f}s excluded; "
+                  f"Gaussians (brute force, C oracle, {t2 - t1:.1f} s); per-Gaussian prep {t1 - t0:.2f}s excluded; "
                   f"extrapolated frame {W * H / (rays / (t2 - t1)) * 1e3:.0f} ms",
-        "seconds": t2 - t1,
     }
 
 
-def run_train(args):
+def traffic_probe(args) -> None:
+    """Child of measure_traffic (runs under rocprofv3 --pmc): a few training
+    steps of the workload, then the calibration kernel of known size."""
     import torch
 
-    from gray_amd import RaytracerConfig, scenes
+    torch.cuda.set_device(0)
+    scene, cam, cfg = train_scene(args.gaussians)
+    rt = setup_raytracer(scene, cam, cfg)
+    target = torch.rand(3, cam.image_height, cam.image_width, generator=torch.Generator().manual_seed(1)).cuda()
+    step = make_step(rt, cam, target)
+    for _ in range(3):
+        step()
+    src = torch.ones(CAL_FLOATS, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+
+
+def _counter_means(csv_path: str, counter: str):
+    kern, cal = [], []
+    with open(csv_path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            name = row["Kernel_Name"]
+            if f"{DOMINANT_KERNEL}(" in name:
+                kern.append(float(row["Counter_Value"]))
+            elif "neg_kernel" in name:
+                cal.append(float(row["Counter_Value"]))
+    if not kern or not cal:
+        return None
+    return sum(kern) / len(kern), sum(cal) / len(cal)
+
+
+def measure_traffic(args):
+    """HBM bytes per launch of the dominant kernel: one rocprofv3 --pmc run
+    per counter (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by
+    what the same counter reads for the calibration kernel of known size."""
+    rocprof = shutil.which("rocprofv3")
+    if not rocprof:
+        return None, "rocprofv3 not found"
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="gray_pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "240", rocprof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o",
+               "p", "--", sys.executable, os.path.abspath(__file__), "--traffic-probe", "--gaussians",
+               str(args.gaussians)]
+        log(f"[bench] traffic: rocprofv3 --pmc {counter}")
+        r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True, text=True)
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        means = _counter_means(files[0], counter) if files else None
+        shutil.rmtree(d, ignore_errors=True)
+        if r.returncode != 0 or means is None:
+            return None, f"rocprofv3 --pmc {counter} gave no result (exit {r.returncode})"
+        out[counter] = means
+    fetch_k, fetch_cal = out["FETCH_SIZE"]
+    write_k, write_cal = out["WRITE_SIZE"]
+    f_fetch, f_write = CAL_KIB / fetch_cal, CAL_KIB / write_cal
+    traffic = (fetch_k * f_fetch + write_k * f_write) * 1024.0
+    return {"bytes": traffic, "fetch_kib_raw": fetch_k, "write_kib_raw": write_k, "fetch_factor": f_fetch,
+            "write_factor": f_write}, None
+
+
+def copy_peak_gbs():
+    """Measured device-to-device copy rate (read + write bytes / time)."""
+    import torch
+
+    n = 256 * 1024 * 1024
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    del a, b
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
+def run_train(args, traffic):
+    import torch
+
     from gray_amd.native import PhaseTimer, _NoTimer
 
-    n, W, H = args.gaussians, 980, 545
+    n = args.gaussians
     log(f"[bench] generating scene: {n} Gaussians on 20 surfaces")
-    scene = scenes.surface_scene(n, 20, -4.5, 0.4, seed=0)
-    cam = scenes.ring_camera(6.0, 0.0, W, H, 60.0)
-    cfg = RaytracerConfig(lr_mean_init=1.6e-4, lr_scale_init=5e-3, lr_sh_dc_init=2.5e-3, lr_sh_rest=2.5e-3)
+    scene, cam, cfg = train_scene(n)
+    W, H = cam.image_width, cam.image_height
     rt = setup_raytracer(scene, cam, cfg)
     target = torch.rand(3, H, W, generator=torch.Generator().manual_seed(1)).cuda()
     nm = rt.cuda_module
-
-    def step():
-        render = rt(cam)
-        loss = torch.nn.functional.l1_loss(render, target)
-        rt.backward(loss)
-        rt.step()
+    step = make_step(rt, cam, target)
 
     log(f"[bench] warmup {args.warmup} steps")
     for _ in range(args.warmup):
@@ -150,11 +255,20 @@ def run_train(args):
     hits = float(stats.num_gaussians_hit.float().sum()) / max(1, (args.warmup + args.steps) * rays)
     acc = float(stats.num_gaussians_accumulated.float().sum()) / max(1, (args.warmup + args.steps) * rays)
 
-    # dominant native kernel and its roofline
-    dom = max(phases, key=lambda k: phases[k][2])
-    launches, mean_ms, _ = phases[dom]
-    alg = BYTES_PER_GAUSSIAN.get(dom, 0) * n + BYTES_PER_RAY.get(dom, 0) * rays
-    achieved = alg / (mean_ms * 1e-3) / 1e9
+    # dominant kernel: HIP events recorded by the library around it, on the
+    # stream it runs on, over further steps of the same workload
+    log(f"[bench] kernel timing: {args.kernel_reps} steps with stage events")
+    nm.set_device_timing(True)
+    stages = []
+    for _ in range(args.kernel_reps):
+        step()
+        stages.append(nm.forward_stage_ms())
+    nm.set_device_timing(False)
+    stage_ms = {k: sum(s[k] for s in stages) / len(stages) for k in stages[0]}
+    launch_ms = stage_ms["collect"]
+    alg = COLLECT_BYTES_PER_GAUSSIAN * n
+    achieved = alg / (launch_ms * 1e-3) / 1e9
+    step_alg = STEP_BYTES_PER_GAUSSIAN * n + STEP_BYTES_PER_RAY * rays
     out = {
         "metric": METRIC,
         "value": rays * args.steps / total_s,
@@ -174,17 +288,22 @@ def run_train(args):
         "step_ms_p50": step_ms[len(step_ms) // 2],
         "step_ms_p99": step_ms[min(len(step_ms) - 1, int(math.ceil(0.99 * len(step_ms))) - 1)],
         "phases_ms": {k: round(v[1], 4) for k, v in phases.items()},
+        "forward_stages_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "hits_per_ray": round(hits, 2),
         "composited_per_ray": round(acc, 2),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "algorithmic_bytes": alg, "launch_ms": round(mean_ms, 4), "launches": launches},
+        "roofline": {"bound": "hbm", "kernel": DOMINANT_KERNEL, "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": round(traffic["bytes"]) if traffic else None,
+                     "algorithmic_bytes": alg, "launch_ms": round(launch_ms, 4),
+                     "traffic_detail": traffic},
+        "step_roofline": {"algorithmic_bytes": step_alg,
+                          "achieved": round(step_alg / (ms_per_step * 1e-3) / 1e9, 2),
+                          "frac": round(step_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                          "measured_copy_GBs": round(copy_peak_gbs(), 1)},
     }
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline on {args.cpu_rays} rays")
-        cb = cpu_baseline(scene, cam, cfg, args.cpu_rays)
-        cb.pop("seconds", None)
-        out["cpu_baseline"] = cb
+        out["cpu_baseline"] = cpu_baseline(scene, cam, cfg, args.cpu_rays)
     else:
         out["cpu_baseline"] = None
     return out
@@ -197,17 +316,26 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="train", choices=["train"])
     ap.add_argument("--gaussians", type=int, default=1_000_000)
-    ap.add_argument("--cpu-rays", type=int, default=1024)
+    ap.add_argument("--cpu-rays", type=int, default=32768)
+    ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the multi-GPU design is described in "
-              "DESIGN.md and is not run.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): every box here has one GPU, and the "
+              "multi-GPU design is documented, not run.", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
+    traffic, why = (None, "skipped") if args.no_traffic else measure_traffic(args)
+    if traffic is None:
+        log(f"[bench] traffic: {why}")
     import torch
 
     torch.cuda.set_device(0)
-    out = run_train(args)
+    out = run_train(args, traffic)
     print(json.dumps(out), flush=True)
 
 

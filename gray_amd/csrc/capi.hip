@@ -40,6 +40,8 @@ extern "C" int gray_ctx_create(gray_ctx **out) {
     if (e == hipSuccess) e = hipMemset(ctx->err, 0, sizeof(int32_t));
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev0);
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
+    for (hipEvent_t &fe : ctx->fev)
+        if (e == hipSuccess) e = hipEventCreate(&fe);
     if (e != hipSuccess) {
         gray_check_hip(e, "ctx_create");
         gray_ctx_destroy(ctx);
@@ -60,6 +62,8 @@ extern "C" void gray_ctx_destroy(gray_ctx *ctx) {
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->fev)
+        if (e) (void)hipEventDestroy(e);
     delete ctx;
 }
 

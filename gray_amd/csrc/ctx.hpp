@@ -51,6 +51,10 @@ struct gray_ctx {
     bool timing = false;
     float last_build_ms = -1.0f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // forward stage events (recorded when timing is on): collect, task rounds,
+    // list compositing, re-traversal
+    hipEvent_t fev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool fev_recorded = false;
 };
 
 int gray_ctx_reserve(gray_ctx *ctx, int64_t n);

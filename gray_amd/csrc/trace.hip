@@ -1639,6 +1639,13 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
         }
         if (A.total_tiles == 0) continue;
         const dim3 grid(8 * A.xcd_chunk), block(GRAY_WAVE);
+        // stage events (first chunk of views only): fev[0..4] bound collect,
+        // task rounds, list compositing and re-traversal
+        const bool timed = ctx->timing && first == 0;
+        auto mark = [&](int i) -> int {
+            if (timed) GRAY_HIP(hipEventRecord(ctx->fev[i], s));
+            return GRAY_OK;
+        };
         if (args->list_capacity > 0 && A.use_bvh && A.num_gaussians > 0) {
             const int64_t lanes = (int64_t)A.total_tiles * GRAY_WAVE;
             if ((rc = reserve_lists(ctx, lanes, args->list_capacity))) return rc;
@@ -1655,8 +1662,10 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             A.task_out = ctx->tasks[0];
             A.task_count_out = count;
             A.budget = args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
+            if ((rc = mark(0))) return rc;
             hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, A.nodes);
             GRAY_HIP(hipGetLastError());
+            if ((rc = mark(1))) return rc;
             for (int round = 0; round < kTaskRounds; ++round) {
                 A.task_in = ctx->tasks[round & 1];
                 A.task_count_in = count + round;
@@ -1673,12 +1682,33 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
                                        A.nodes);
                 GRAY_HIP(hipGetLastError());
             }
+            if ((rc = mark(2))) return rc;
             hipLaunchKernelGGL(k_composite_lists, grid, block, 0, s, A, A.views[0].rgb);
             GRAY_HIP(hipGetLastError());
+        } else {
+            for (int i = 0; i < 3; ++i)
+                if ((rc = mark(i))) return rc;
         }
+        if ((rc = mark(3))) return rc;
         hipLaunchKernelGGL(k_trace_forward, grid, block, 0, s, A, A.geo, A.nodes, A.views[0].rgb);
         GRAY_HIP(hipGetLastError());
+        if ((rc = mark(4))) return rc;
+        if (timed) ctx->fev_recorded = true;
     }
+    return GRAY_OK;
+}
+
+extern "C" int gray_render_stage_ms(gray_ctx *ctx, float *ms, int32_t max_stages) {
+    if (!ctx || !ms || max_stages < 4) {
+        gray_set_error(GRAY_ERR_INVALID, "render_stage_ms: null pointer or fewer than 4 stages");
+        return GRAY_ERR_INVALID;
+    }
+    if (!ctx->fev_recorded) {
+        gray_set_error(GRAY_ERR_STATE, "render_stage_ms: no timed forward (enable timing first)");
+        return GRAY_ERR_STATE;
+    }
+    GRAY_HIP(hipEventSynchronize(ctx->fev[4]));
+    for (int i = 0; i < 4; ++i) GRAY_HIP(hipEventElapsedTime(&ms[i], ctx->fev[i], ctx->fev[i + 1]));
     return GRAY_OK;
 }
 

@@ -560,6 +560,20 @@ class NativeRaytracer:
                                      self._stream()))
         return nodes, ids, keys
 
+    def set_device_timing(self, enable: bool) -> None:
+        """Record HIP events around BVH rebuilds and the forward's stages."""
+        L.check(L.LIB.gray_bvh_set_timing(self.ctx.handle, 1 if enable else 0))
+
+    def last_build_ms(self) -> float:
+        """Device milliseconds of the last timed BVH rebuild (-1: none)."""
+        return float(L.LIB.gray_bvh_last_build_ms(self.ctx.handle))
+
+    def forward_stage_ms(self) -> dict:
+        """Device milliseconds of the last timed forward's stages; waits for it."""
+        ms = (C.c_float * 4)()
+        L.check(L.LIB.gray_render_stage_ms(self.ctx.handle, ms, 4))
+        return {"collect": ms[0], "collect_tasks": ms[1], "composite_lists": ms[2], "retraverse": ms[3]}
+
     def debug_list_counts(self):
         """Hits per ray of the last forward that collected lists, int32
         [tiles*64] in tile order (lane = 8*(y%8) + x%8; -1 = re-traversed)."""

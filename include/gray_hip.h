@@ -191,9 +191,17 @@ int gray_bvh_update(gray_ctx *ctx, const gray_params *params, const gray_knobs *
                     void *stream);
 
 /* Milliseconds of the last rebuild (device time, measured with HIP events),
- * or -1 if none was timed.  Timing is only recorded when enabled. */
+ * or -1 if none was timed.  Timing is only recorded when enabled; it also
+ * records the forward's stage events (see gray_render_stage_ms). */
 int gray_bvh_set_timing(gray_ctx *ctx, int enable);
 float gray_bvh_last_build_ms(gray_ctx *ctx);
+
+/* No reference counterpart (profiling aid).  Device milliseconds of the
+ * stages of the last timed gray_render_forward (its first 8 views): ms[0] the
+ * per-tile collect kernel, ms[1] the collect task rounds, ms[2] list
+ * compositing, ms[3] the re-traversal forward (stages a launch skipped read
+ * ~0).  Waits for that forward to finish.  max_stages >= 4. */
+int gray_render_stage_ms(gray_ctx *ctx, float *ms, int32_t max_stages);
 
 /* Number of Gaussians and internal nodes of the current BVH. */
 int gray_bvh_info(gray_ctx *ctx, int64_t *num_gaussians, int64_t *num_nodes, int64_t *depth_bound);
