@@ -10,6 +10,8 @@ enter the detached truncation estimate and the full transmittance).
 """
 from __future__ import annotations
 
+import math
+
 import torch
 
 MAX_ALPHA = 0.9999
@@ -26,9 +28,52 @@ def build_rotation(q: torch.Tensor) -> torch.Tensor:
     return R.reshape(-1, 3, 3)
 
 
+def sh_basis(x, y, z):
+    """Real SH of degrees 1-3 in closed form (the 15 non-DC terms, in the
+    order of the reference's coefficients, sh.cu:11-78)."""
+    pi = math.pi
+    c1 = math.sqrt(3 / (4 * pi))
+    c2a, c2b, c2c = math.sqrt(15 / (4 * pi)), math.sqrt(5 / (16 * pi)), math.sqrt(15 / (16 * pi))
+    c3a, c3b, c3c = math.sqrt(35 / (32 * pi)), math.sqrt(105 / (4 * pi)), math.sqrt(21 / (32 * pi))
+    c3d, c3e = math.sqrt(7 / (16 * pi)), math.sqrt(105 / (16 * pi))
+    xx, yy, zz = x * x, y * y, z * z
+    return [
+        -c1 * y, c1 * z, -c1 * x,
+        c2a * x * y, -c2a * y * z, c2b * (2 * zz - xx - yy), -c2a * x * z, c2c * (xx - yy),
+        -c3a * y * (3 * xx - yy), c3b * x * y * z, -c3c * y * (4 * zz - xx - yy),
+        c3d * z * (2 * zz - 3 * xx - 3 * yy), -c3c * x * (4 * zz - xx - yy), c3e * z * (xx - yy),
+        -c3a * x * (xx - 3 * yy),
+    ]
+
+
+def sh_colors(mean, sh_dc, sh_rest, origin):
+    """Degree-3 colours max(C0 dc + sum Y_k c_k + 0.5, 0) seen from origin
+    (sh.cu:224-247); differentiable in mean, sh_dc [N,1,3], sh_rest [N,15,3]."""
+    delta = mean - origin
+    d = delta / delta.norm(dim=1, keepdim=True)
+    B = torch.stack([torch.as_tensor(v) for v in sh_basis(d[:, 0], d[:, 1], d[:, 2])], 1)
+    return (0.28209479177387814 * sh_dc[:, 0] + (B[:, :, None] * sh_rest).sum(1) + 0.5).clamp(min=0)
+
+
 def clip_factor(op: torch.Tensor, athr: float, p: float) -> torch.Tensor:
     lr = torch.log(op / athr)
     return torch.where(op > athr, (2 * p * lr.clamp(min=0)) ** (1.0 / (2 * p)), torch.zeros_like(op))
+
+
+def hit_mask(mean, rotation, scale, opacity, ro, rd, exp_power=2.0, alpha_threshold=0.01):
+    """[R, N] bool: which Gaussians each ray hits (the criterion of render)."""
+    with torch.no_grad():
+        op = torch.sigmoid(opacity.reshape(-1))
+        clip = clip_factor(op, alpha_threshold, exp_power)
+        q = rotation / rotation.norm(dim=-1, keepdim=True)
+        W2L = torch.linalg.inv(build_rotation(q) * torch.exp(scale)[:, None, :])
+        lo = torch.einsum("nij,rnj->rni", W2L, ro[:, None, :] - mean[None, :, :])
+        ld = torch.einsum("nij,rj->rni", W2L, rd)
+        ldn = ld / ld.norm(dim=-1)[..., None]
+        hit_local = lo + (-(lo * ldn).sum(-1))[..., None] * ldn
+        sq_clip = (hit_local ** 2).sum(-1) / (clip[None, :] ** 2).clamp(min=1e-300)
+        behind = (lo * ld).sum(-1) > 0
+        return (clip > 0)[None, :] & (~behind) & (sq_clip <= 1.0) & (op > alpha_threshold)[None, :]
 
 
 def render(mean, rotation, scale, opacity, colors, ro, rd, exp_power=2.0, alpha_threshold=0.01, t_threshold=0.01,

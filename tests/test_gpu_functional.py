@@ -1,0 +1,150 @@
+"""Multi-camera autograd rendering (gray_amd.functional) on the GPU: a batch of
+views equals the same views rendered one at a time, its gradients equal the
+per-view oracle gradients summed, and they match float64 central finite
+differences (h = 1e-4) of an independent restatement (oracle/torch_ref.py)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from gray_amd import scenes
+from gray_amd.functional import MultiViewRenderer, RenderSettings
+from oracle import oracle as O
+from oracle import torch_ref
+
+from gpu_util import rel_err, setup_native
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = ("mean", "rotation", "scale", "opacity", "sh_dc", "sh_rest")
+
+
+def scene_tensors(scene, requires_grad=True):
+    return [getattr(scene, k).cuda().clone().requires_grad_(requires_grad) for k in PARAMS]
+
+
+def ring(count, w, h, radius=3.0):
+    return [scenes.ring_camera(radius, 2 * math.pi * i / count + 0.3, w, h, 60.0, uid=i) for i in range(count)]
+
+
+@pytest.mark.parametrize("views", [3, 10])
+def test_batch_equals_single_views(views):
+    scene = scenes.tiny_scene(1024, seed=4)
+    cams = ring(views, 40, 32)
+    r = MultiViewRenderer(40, 32, RenderSettings(t_threshold=0.03))
+    with torch.no_grad():
+        imgs = r.render(cams, *scene_tensors(scene, False))
+    torch.cuda.synchronize()
+    r.device_status()
+    for i, cam in enumerate(cams):
+        nr = setup_native(scene, cam, t_threshold=0.03)
+        nr.forward_pass()
+        single = nr.get_framebuffer().output_channels[:32, :40]
+        assert (imgs[i] - single).abs().max().item() < 1e-6, i
+
+
+def test_gradients_equal_summed_oracle_views():
+    scene = scenes.tiny_scene(1024, seed=5)
+    cams = ring(3, 32, 24)
+    r = MultiViewRenderer(32, 24, RenderSettings(t_threshold=0.03))
+    params = scene_tensors(scene)
+    imgs = r.render(cams, *params)
+    w = torch.randn(imgs.shape, generator=torch.Generator().manual_seed(2)).cuda()
+    (imgs * w).sum().backward()
+    torch.cuda.synchronize()
+    r.device_status()
+    k = O.make_knobs(t_threshold=0.03)
+    geo, l2w = O.snapshot(scene.mean.numpy(), scene.rotation.numpy(), scene.scale.numpy(), scene.opacity.numpy(),
+                          0.01, 2.0)
+    tot = {key: 0 for key in ("mean", "rotation", "scale", "opacity", "sh_dc", "sh_rest")}
+    for i, cam in enumerate(cams):
+        ro = r.ro[i].reshape(-1, 3).cpu().numpy()
+        rd = r.rd[i].reshape(-1, 3).cpu().numpy()
+        origin = cam.origin.astype(np.float32)
+        rgb = O.sh_forward(scene.mean.numpy(), scene.sh_dc.numpy(), scene.sh_rest.numpy(), origin, 3)
+        fwd = O.render(geo, rgb, ro, rd, k)
+        assert np.abs(imgs[i].detach().reshape(-1, 3).cpu().numpy() - fwd["C"]).max() < 1e-5
+        b = O.backward(geo, l2w, rgb, scene.rotation.numpy(), scene.scale.numpy(), scene.opacity.numpy(), ro, rd,
+                       fwd, w[i].reshape(-1, 3).cpu().numpy(), k, 32 * 24)
+        vis = ((np.abs(b["opacity"]).reshape(-1) > 0) | (b["pruning"].reshape(-1) > 0)).astype(np.uint8)
+        gdc, grest, gmean = O.sh_backward(scene.mean.numpy(), scene.sh_rest.numpy(), origin, rgb, b["rgb"], vis, 3,
+                                          grad_mean=b["mean"])
+        tot["mean"] = tot["mean"] + gmean
+        tot["rotation"] = tot["rotation"] + b["rotation"]
+        tot["scale"] = tot["scale"] + b["scale"]
+        tot["opacity"] = tot["opacity"] + b["opacity"]
+        tot["sh_dc"] = tot["sh_dc"] + gdc
+        tot["sh_rest"] = tot["sh_rest"] + grest
+    for name, p in zip(PARAMS, params):
+        assert rel_err(p.grad.cpu(), tot[name].reshape(p.shape)) < 1e-3, name
+
+
+def test_gradients_match_fp64_finite_differences():
+    """t_threshold 0 (every hit composited): the forward is then smooth in every
+    parameter away from the hit-criterion boundary, and the reference backward
+    is its exact derivative.  Parameters whose +-h perturbation changes any
+    ray's hit set sit on that boundary and are skipped."""
+    scene = scenes.tiny_scene(300, seed=6)
+    W = H = 24
+    cams = ring(4, W, H, radius=2.5)
+    s = RenderSettings(t_threshold=0.0)
+    r = MultiViewRenderer(W, H, s)
+    params = scene_tensors(scene)
+    imgs = r.render(cams, *params)
+    wgt = torch.randn(imgs.shape, generator=torch.Generator().manual_seed(3)).cuda()
+    (imgs * wgt).sum().backward()
+    torch.cuda.synchronize()
+    r.device_status()
+    ro = torch.tensor(r.ro.reshape(4, -1, 3).cpu().numpy(), dtype=torch.float64)
+    rd = torch.tensor(r.rd.reshape(4, -1, 3).cpu().numpy(), dtype=torch.float64)
+    wv = torch.tensor(wgt.reshape(4, -1, 3).cpu().numpy(), dtype=torch.float64)
+    p64 = [getattr(scene, k).to(torch.float64) for k in PARAMS]
+    origins = [torch.tensor(c.origin, dtype=torch.float64) for c in cams]
+
+    def masks(pp, j):
+        sl = [t[j:j + 1] for t in pp[:4]]
+        return [torch_ref.hit_mask(*sl, ro[v], rd[v])[:, 0] for v in range(4)]
+
+    def loss(pp, rays):
+        tot = torch.zeros((), dtype=torch.float64)
+        for v in range(4):
+            idx = rays[v]
+            if idx.numel() == 0:
+                continue
+            col = torch_ref.sh_colors(pp[0], pp[4], pp[5], origins[v])
+            img, _ = torch_ref.render(pp[0], pp[1], pp[2], pp[3], col, ro[v][idx], rd[v][idx], 2.0, 0.01, 0.0)
+            tot = tot + (img * wv[v][idx]).sum()
+        return tot
+
+    rng = np.random.default_rng(7)
+    seen = torch.zeros(300, dtype=torch.bool)
+    for v in range(4):
+        seen |= torch_ref.hit_mask(*p64[:4], ro[v], rd[v]).any(0)
+    cand = np.nonzero(seen.numpy())[0]
+    h = 1e-4
+    checked, worst = 0, 0.0
+    for _ in range(400):
+        if checked == 64:
+            break
+        j = int(rng.choice(cand))
+        which = int(rng.integers(0, 6))
+        comp = int(rng.integers(0, p64[which][j].numel()))
+        base = masks(p64, j)
+        pert = []
+        for sign in (1.0, -1.0):
+            pp = [t.clone() for t in p64]
+            pp[which][j].view(-1)[comp] += sign * h
+            pert.append(pp)
+        if any(not torch.equal(a, b) for pp in pert for a, b in zip(masks(pp, j), base)):
+            continue  # on the hit-criterion boundary
+        rays = [torch.nonzero(m).reshape(-1) for m in base]
+        fd = float(loss(pert[0], rays) - loss(pert[1], rays)) / (2 * h)
+        g = float(params[which].grad[j].reshape(-1)[comp])
+        ref = float(params[which].grad.abs().max())
+        err = abs(g - fd) / (abs(fd) + 1e-3 * ref + 1e-9)
+        worst = max(worst, err)
+        assert err < 1e-3, (PARAMS[which], j, comp, g, fd)
+        checked += 1
+    assert checked == 64, checked
+    print(f"64 parameters, worst relative error {worst:.2e}")

@@ -18,20 +18,7 @@ MAX_ALPHA = 0.9999
 # ---------------------------------------------------------------------------
 # spherical harmonics: reference complex recursion vs closed form
 # ---------------------------------------------------------------------------
-def closed_form_basis(x, y, z):
-    pi = math.pi
-    c1 = math.sqrt(3 / (4 * pi))
-    c2a, c2b, c2c = math.sqrt(15 / (4 * pi)), math.sqrt(5 / (16 * pi)), math.sqrt(15 / (16 * pi))
-    c3a, c3b, c3c = math.sqrt(35 / (32 * pi)), math.sqrt(105 / (4 * pi)), math.sqrt(21 / (32 * pi))
-    c3d, c3e = math.sqrt(7 / (16 * pi)), math.sqrt(105 / (16 * pi))
-    xx, yy, zz = x * x, y * y, z * z
-    return [
-        -c1 * y, c1 * z, -c1 * x,
-        c2a * x * y, -c2a * y * z, c2b * (2 * zz - xx - yy), -c2a * x * z, c2c * (xx - yy),
-        -c3a * y * (3 * xx - yy), c3b * x * y * z, -c3c * y * (4 * zz - xx - yy),
-        c3d * z * (2 * zz - 3 * xx - 3 * yy), -c3c * x * (4 * zz - xx - yy), c3e * z * (xx - yy),
-        -c3a * x * (xx - 3 * yy),
-    ]
+closed_form_basis = torch_ref.sh_basis
 
 
 def test_closed_form_is_orthonormal():

@@ -57,3 +57,14 @@ def test_holder_resize_clears_new_rows():
         g.resize(4)
     assert hit_capacity_for(980 * 545, 120_000_000) == 64
     assert hit_capacity_for(4096, 0) == 0
+
+
+def test_capacities_hold_the_cap():
+    from gray_amd.functional import capacities
+
+    for views, w, h in ((8, 512, 512), (1, 980, 545), (3, 64, 64)):
+        stored, lists = capacities(views, w, h, 4 << 30)
+        tiles = -(-w // 8) * -(-h // 8)
+        lanes = min(views, 8) * tiles * 64
+        used = 8 * stored * views * w * h + 16 * lists * lanes + 256 * lanes
+        assert used <= 4 << 30 and lists > 0
