@@ -230,6 +230,13 @@ class MultiViewRenderer:
                                        stream))
         return g_mean, g_rot, g_scale, g_op, g_dc, g_rest
 
+    def debug_snapshot(self):
+        """Geometry snapshot of the last forward: geo [N,16], aabb [N,6]."""
+        geo = torch.zeros(self._n, 16, dtype=F32, device=self.device)
+        aabb = torch.zeros(self._n, 6, dtype=F32, device=self.device)
+        L.check(L.LIB.gray_debug_snapshot(self.ctx.handle, geo.data_ptr(), aabb.data_ptr(), L.stream_ptr()))
+        return geo, aabb
+
     def device_status(self) -> None:
         L.check(L.LIB.gray_device_status(self.ctx.handle, L.stream_ptr()))
 

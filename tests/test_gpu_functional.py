@@ -148,3 +148,110 @@ def test_gradients_match_fp64_finite_differences():
         checked += 1
     assert checked == 64, checked
     print(f"64 parameters, worst relative error {worst:.2e}")
+
+
+def test_config3_scale_under_memory_cap():
+    """200,000 Gaussians (surface generator), 8 cameras at 512x512 in one
+    launch, hit lists under a 4 GB cap: composited lists longer than the
+    derived store are recomputed in the backward.  Gradients equal those of an
+    uncapped store, and 64 random parameters match float64 central
+    differences (h = 1e-4; t_threshold 0 as above)."""
+    scene = scenes.surface_scene(200_000, 20, -4.5, 0.4, seed=1)
+    cams = scenes.ring_cameras(8, 6.0, 512, 512)
+    capped = MultiViewRenderer(512, 512, RenderSettings(t_threshold=0.0, memory_cap_bytes=4 << 30))
+    params = scene_tensors(scene)
+    imgs = capped.render(cams, *params)
+    wgt = torch.randn(imgs.shape, generator=torch.Generator().manual_seed(9)).cuda()
+    (imgs * wgt).sum().backward()
+    torch.cuda.synchronize()
+    capped.device_status()
+    stored, lists = capped.last_capacities
+    ncomp = capped.n_comp.max().item()
+    assert ncomp > stored, "the cap must force recomputed lists for this check"
+    grads = [p.grad.clone() for p in params]
+    roomy = MultiViewRenderer(512, 512, RenderSettings(t_threshold=0.0, memory_cap_bytes=64 << 30,
+                                                       hit_capacity=64, list_capacity=512))
+    params2 = scene_tensors(scene)
+    (roomy.render(cams, *params2) * wgt).sum().backward()
+    torch.cuda.synchronize()
+    for name, a, b in zip(PARAMS, grads, params2):
+        assert rel_err(a.cpu(), b.grad.cpu()) < 1e-4, name
+    # float64 central differences on 64 random parameters
+    _, aabb = capped.debug_snapshot()
+    aabb = aabb.cpu().numpy().astype(np.float64)
+    ro = torch.tensor(capped.ro.reshape(-1, 3).cpu().numpy(), dtype=torch.float64)
+    rd = torch.tensor(capped.rd.reshape(-1, 3).cpu().numpy(), dtype=torch.float64)
+    wv = torch.tensor(wgt.reshape(-1, 3).cpu().numpy(), dtype=torch.float64)
+    view_of = torch.arange(ro.shape[0]) // (512 * 512)
+    p64 = [getattr(scene, k).to(torch.float64) for k in PARAMS]
+    origins = [torch.tensor(c.origin, dtype=torch.float64) for c in cams]
+
+    # box prefilters in float64 on the device (test scaffolding only)
+    ro_g, inv_g = ro.cuda(), 1.0 / rd.cuda()
+    aabb_g = torch.tensor(aabb, device="cuda")
+
+    def slab(rays, boxes, pad):
+        """[len(rays), len(boxes)]: the ray enters the box grown by pad."""
+        o, inv = ro_g[rays][:, None, :], inv_g[rays][:, None, :]
+        t0 = (boxes[None, :, :3] - pad - o) * inv
+        t1 = (boxes[None, :, 3:] + pad - o) * inv
+        tf = torch.maximum(t0, t1).amin(-1)
+        return (torch.minimum(t0, t1).amax(-1) <= tf) & (tf >= 0)
+
+    all_rays = torch.arange(ro.shape[0], device="cuda")
+
+    def candidates(rays):
+        """Gaussians whose (conservative) box one of the rays enters."""
+        hit = torch.zeros(aabb.shape[0], dtype=torch.bool, device="cuda")
+        for c in range(0, len(rays), 8):
+            hit |= slab(rays[c:c + 8].cuda(), aabb_g, 1e-3).any(0)
+        return torch.nonzero(hit).reshape(-1).cpu().numpy()
+
+    def loss(pp, rays, sub):
+        tot = torch.zeros((), dtype=torch.float64)
+        q = [t[sub] for t in pp]
+        for v in range(8):
+            idx = rays[view_of[rays] == v]
+            if idx.numel() == 0:
+                continue
+            col = torch_ref.sh_colors(q[0], q[4], q[5], origins[v])
+            img, _ = torch_ref.render(q[0], q[1], q[2], q[3], col, ro[idx], rd[idx], 2.0, 0.01, 0.0)
+            tot = tot + (img * wv[idx]).sum()
+        return tot
+
+    rng = np.random.default_rng(11)
+    vis = np.nonzero(params[3].grad.abs().reshape(-1).cpu().numpy() > 0)[0]
+    h = 1e-4
+    checked, worst = 0, 0.0
+    for _ in range(400):
+        if checked == 64:
+            break
+        j = int(rng.choice(vis))
+        which = int(rng.integers(0, 6))
+        comp = int(rng.integers(0, p64[which][j].numel()))
+        near = torch.nonzero(slab(all_rays, aabb_g[j:j + 1], 1e-3)[:, 0]).reshape(-1).cpu()
+        if near.numel() == 0:
+            continue
+        one = lambda pp: torch_ref.hit_mask(*[t[j:j + 1] for t in pp[:4]], ro[near], rd[near])[:, 0]  # noqa: E731
+        base = one(p64)
+        pert = []
+        for sign in (1.0, -1.0):
+            pp = [t.clone() for t in p64]
+            pp[which][j].view(-1)[comp] += sign * h
+            pert.append(pp)
+        if any(not torch.equal(one(pp), base) for pp in pert):
+            continue
+        rays = near[base]
+        if rays.numel() == 0 or rays.numel() > 64:
+            continue
+        sub = torch.tensor(np.union1d(candidates(rays), [j]), dtype=torch.long)
+        fd = float(loss(pert[0], rays, sub) - loss(pert[1], rays, sub)) / (2 * h)
+        g = float(params[which].grad[j].reshape(-1)[comp])
+        ref = float(params[which].grad.abs().max())
+        err = abs(g - fd) / (abs(fd) + 1e-3 * ref + 1e-9)
+        worst = max(worst, err)
+        assert err < 1e-3, (PARAMS[which], j, comp, g, fd)
+        checked += 1
+    assert checked == 64, checked
+    print(f"config 3: capacities stored {stored} / lists {lists}, max composited {ncomp}; "
+          f"64 parameters, worst relative error {worst:.2e}")
