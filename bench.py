@@ -309,13 +309,156 @@ def run_train(args, traffic):
     return out
 
 
+def _median_ms(fn, reps):
+    import torch
+
+    times = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    times.sort()
+    return times[len(times) // 2], times
+
+
+def run_render(args):
+    """Render-only: 3M Gaussians (60 surfaces, log-scale N(-5, 0.3^2)), SH
+    degree 3, 1600x1063, 120 poses on a ring of radius 6.  Batched launches of
+    8 views (MultiViewRenderer) against one camera per launch (the drop-in
+    Raytracer path); BVH build ms with device events."""
+    import torch
+
+    from gray_amd import scenes
+    from gray_amd.functional import MultiViewRenderer, RenderSettings
+
+    n = args.gaussians or 3_000_000
+    W, H, poses = 1600, 1063, 120
+    log(f"[bench] render: generating {n} Gaussians on 60 surfaces")
+    scene = scenes.surface_scene(n, 60, -5.0, 0.3, seed=0)
+    cams = scenes.ring_cameras(poses, 6.0, W, H)
+    params = [getattr(scene, k).cuda() for k in ("mean", "rotation", "scale", "opacity", "sh_dc", "sh_rest")]
+    # plain: one camera per launch through the drop-in API
+    from gray_amd import Raytracer, RaytracerConfig
+
+    rt = setup_raytracer(scene, cams[0], RaytracerConfig())
+    nm = rt.cuda_module
+    nm.set_device_timing(True)
+    builds = []
+    for _ in range(3):
+        nm.rebuild_bvh()
+        torch.cuda.synchronize()
+        builds.append(nm.last_build_ms())
+    nm.set_device_timing(False)
+    build_ms = sorted(builds)[1]
+    with torch.no_grad():
+        for c in cams[:2]:
+            rt(c)
+        torch.cuda.synchronize()
+
+        def plain():
+            for c in cams:
+                rt(c)
+
+        plain_ms, _ = _median_ms(plain, 3)
+    nm.device_status()
+    del rt, nm
+    torch.cuda.empty_cache()
+    # batched: 8 views per launch, lists under a memory cap, no stored lists
+    lanes = 8 * (-(-W // 8)) * (-(-H // 8)) * 64
+    list_cap = int(min(512, args.render_list_gb * (1 << 30) / (16 * lanes)))
+    r = MultiViewRenderer(W, H, RenderSettings(hit_capacity=0, list_capacity=list_cap))
+    with torch.no_grad():
+        r.render(cams[:8], *params, rebuild=True)
+        torch.cuda.synchronize()
+
+        def batched():
+            for i in range(0, poses, 8):
+                r.render(cams[i:i + 8], *params)
+
+        batched_ms, _ = _median_ms(batched, 3)
+    r.device_status()
+    fps = poses / (batched_ms * 1e-3)
+    alg = 236 * n + 12 * W * H
+    return {
+        "metric": "render-only frames/s at 3M Gaussians (1600x1063, 120 poses)",
+        "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": poses, "warmup": 8,
+        "ms_per_step": batched_ms / poses, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded surface scene)",
+        "config": {"workload": "render_3M_1600x1063_120poses", "gaussians": n, "width": W, "height": H,
+                   "poses": poses, "views_per_launch": 8, "list_capacity": list_cap},
+        "rays_per_s": W * H * fps,
+        "plain_launch": {"frames_per_s": poses / (plain_ms * 1e-3), "ms_per_frame": plain_ms / poses},
+        "bvh_build_ms": build_ms,
+        "hbm_algorithmic_GBs": alg * fps / 1e9,
+        "roofline": {"bound": "hbm", "kernel": "frame", "achieved": round(alg * fps / 1e9, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg * fps / 1e9 / HBM_PEAK_GBS, 5), "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
+def run_fwdbwd(args):
+    """Differentiation workload: 200K Gaussians (surface generator), 8 cameras
+    at 512x512 in one launch, forward + backward through the autograd
+    Function with hit lists under a 4 GB cap, against 8 single-camera calls."""
+    import torch
+
+    from gray_amd import scenes
+    from gray_amd.functional import MultiViewRenderer, RenderSettings
+
+    n = args.gaussians or 200_000
+    W = H = 512
+    scene = scenes.surface_scene(n, 20, -4.5, 0.4, seed=1)
+    cams = scenes.ring_cameras(8, 6.0, W, H)
+    params = [getattr(scene, k).cuda().clone().requires_grad_(True)
+              for k in ("mean", "rotation", "scale", "opacity", "sh_dc", "sh_rest")]
+    wgt = torch.randn(8, H, W, 3, generator=torch.Generator().manual_seed(9)).cuda()
+    batch = MultiViewRenderer(W, H, RenderSettings(memory_cap_bytes=4 << 30))
+    single = MultiViewRenderer(W, H, RenderSettings(memory_cap_bytes=4 << 30))
+
+    def run_batch():
+        (batch.render(cams, *params) * wgt).sum().backward()
+
+    def run_single():
+        for i in range(8):
+            (single.render(cams[i:i + 1], *params) * wgt[i:i + 1]).sum().backward()
+
+    for fn in (run_batch, run_single):
+        fn()
+    torch.cuda.synchronize()
+    b_ms, _ = _median_ms(run_batch, 10)
+    s_ms, _ = _median_ms(run_single, 10)
+    batch.device_status()
+    stored, lists = batch.last_capacities
+    tiles = (-(-W // 8)) * (-(-H // 8))
+    peak = 8 * stored * 8 * W * H + 16 * lists * 8 * tiles * 64 + 256 * 8 * tiles * 64
+    alg = 3 * 236 * n + 24 * 8 * W * H
+    return {
+        "metric": "forward+backward ms, 8 cameras 512x512 in one launch (200K Gaussians)",
+        "value": b_ms, "unit": "ms", "n_gpus": 1, "steps": 10, "warmup": 1, "ms_per_step": b_ms,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded surface scene, random loss weights)",
+        "config": {"workload": "fwdbwd_200K_8x512x512", "gaussians": n, "views": 8, "width": W, "height": H,
+                   "memory_cap_bytes": 4 << 30, "stored_hits_per_ray": stored, "list_hits_per_ray": lists},
+        "plain_launch": {"ms": s_ms, "note": "8 single-camera forward+backward calls"},
+        "peak_hit_list_bytes": peak,
+        "roofline": {"bound": "hbm", "kernel": "fwd+bwd launch", "achieved": round(alg / (b_ms * 1e-3) / 1e9, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5), "traffic": None},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="train", choices=["train"])
-    ap.add_argument("--gaussians", type=int, default=1_000_000)
+    ap.add_argument("--workload", default="train", choices=["train", "render", "fwdbwd"])
+    ap.add_argument("--gaussians", type=int, default=None, help="default: the workload's own size")
+    ap.add_argument("--render-list-gb", type=float, default=48.0, help="render: hit-list memory of a launch")
     ap.add_argument("--cpu-rays", type=int, default=32768)
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
@@ -326,6 +469,15 @@ def main():
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): every box here has one GPU, and the "
               "multi-GPU design is documented, not run.", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "train":
+        import torch
+
+        torch.cuda.set_device(0)
+        out = run_render(args) if args.workload == "render" else run_fwdbwd(args)
+        print(json.dumps(out), flush=True)
+        return
+    if args.gaussians is None:
+        args.gaussians = 1_000_000
     if args.traffic_probe:
         traffic_probe(args)
         return
