@@ -451,12 +451,64 @@ def run_fwdbwd(args):
     }
 
 
+def run_serve(args):
+    """Serving: 2M Gaussians resident (60 surfaces, log-scale N(-5, 0.3^2)),
+    Poisson arrivals of camera requests (sphere of radius 4-8, resolutions
+    uniform over 256^2, 512x384, 1280x720, 1600x900) at 0.5x, 1x and 2x the
+    measured single-stream capacity; continuous batching against one launch
+    per request on the same GPU."""
+    import torch
+
+    from gray_amd import scenes
+    from gray_amd.serving import SceneServer, poisson_requests
+
+    n = args.gaussians or 2_000_000
+    log(f"[bench] serve: generating {n} Gaussians on 60 surfaces")
+    scene = scenes.surface_scene(n, 60, -5.0, 0.3, seed=0)
+    t = [getattr(scene, k).cuda() for k in ("mean", "rotation", "scale", "opacity", "sh_dc", "sh_rest")]
+    server = SceneServer(*t, slots=args.serve_slots, budget_bytes=int(args.serve_budget_gb * (1 << 30)),
+                         list_capacity=256)
+    server.serve(poisson_requests(16, 0.0, seed=99), batched=True)  # warm-up
+    server.serve(poisson_requests(8, 0.0, seed=98), batched=False)
+    cap_stats = server.serve(poisson_requests(48, 0.0, seed=1), batched=False)
+    capacity = cap_stats["frames_per_s"]
+    log(f"[bench] serve: single-stream capacity {capacity:.1f} requests/s")
+    rows = []
+    for i, mult in enumerate((0.5, 1.0, 2.0)):
+        rate = mult * capacity
+        row = {"rate_x_capacity": mult, "rate_per_s": rate}
+        for mode, batched in (("batched", True), ("plain", False)):
+            reqs = poisson_requests(args.serve_requests, rate, seed=100 + i)
+            st = server.serve(reqs, batched=batched)
+            row[mode] = {k: round(v, 3) for k, v in st.items()}
+            log(f"[bench] serve {mult}x {mode}: p50 {st['p50_ms']:.1f} ms p99 {st['p99_ms']:.1f} ms "
+                f"{st['frames_per_s']:.1f} frames/s (mean batch {st['mean_batch']:.2f})")
+        rows.append(row)
+    top = rows[-1]["batched"]
+    return {
+        "metric": "served p50/p99 frame latency and frames/s vs plain launch (2M Gaussians, mixed resolutions)",
+        "value": top["frames_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": args.serve_requests, "warmup": 24,
+        "ms_per_step": 1e3 / top["frames_per_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded surface scene, seeded Poisson requests)",
+        "config": {"workload": "serve_2M_mixed_poisson", "gaussians": n, "slots": args.serve_slots,
+                   "hbm_budget_gb": args.serve_budget_gb, "list_capacity": 256,
+                   "resolutions": ["256x256", "512x384", "1280x720", "1600x900"]},
+        "single_stream_capacity_per_s": capacity,
+        "rates": rows,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="train", choices=["train", "render", "fwdbwd"])
+    ap.add_argument("--workload", default="train", choices=["train", "render", "fwdbwd", "serve"])
+    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-budget-gb", type=float, default=48.0)
     ap.add_argument("--gaussians", type=int, default=None, help="default: the workload's own size")
     ap.add_argument("--render-list-gb", type=float, default=48.0, help="render: hit-list memory of a launch")
     ap.add_argument("--cpu-rays", type=int, default=32768)
@@ -473,7 +525,7 @@ def main():
         import torch
 
         torch.cuda.set_device(0)
-        out = run_render(args) if args.workload == "render" else run_fwdbwd(args)
+        out = {"render": run_render, "fwdbwd": run_fwdbwd, "serve": run_serve}[args.workload](args)
         print(json.dumps(out), flush=True)
         return
     if args.gaussians is None:

@@ -110,6 +110,7 @@ def _load():
         "gray_debug_bvh": (i32, [vp, vp, vp, vp, vp]),
         "gray_debug_list_counts": (i32, [vp, vp, i64, C.POINTER(i64), vp]),
         "gray_render_stage_ms": (i32, [vp, C.POINTER(C.c_float), i32]),
+        "gray_render_reserve": (i32, [vp, i64, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -128,6 +129,7 @@ EXPORTED = (
     "gray_bvh_update", "gray_bvh_set_timing", "gray_bvh_last_build_ms", "gray_bvh_info", "gray_sh_forward",
     "gray_sh_backward", "gray_render_forward", "gray_render_backward", "gray_adam_step", "gray_debug_hitlists",
     "gray_debug_snapshot", "gray_debug_bvh", "gray_debug_list_counts", "gray_render_stage_ms",
+    "gray_render_reserve",
 )
 
 

@@ -1712,6 +1712,15 @@ extern "C" int gray_render_stage_ms(gray_ctx *ctx, float *ms, int32_t max_stages
     return GRAY_OK;
 }
 
+extern "C" int gray_render_reserve(gray_ctx *ctx, int64_t max_lanes, int32_t list_capacity) {
+    if (!ctx || max_lanes < 0 || list_capacity < 0) {
+        gray_set_error(GRAY_ERR_INVALID, "render_reserve: null context or negative size");
+        return GRAY_ERR_INVALID;
+    }
+    if (max_lanes == 0 || list_capacity == 0) return GRAY_OK;
+    return reserve_lists(ctx, max_lanes, list_capacity);
+}
+
 extern "C" int gray_debug_list_counts(gray_ctx *ctx, int32_t *counts, int64_t max_lanes, int64_t *lanes,
                                       void *stream) {
     if (!ctx || !lanes) {

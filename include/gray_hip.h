@@ -196,6 +196,13 @@ int gray_bvh_update(gray_ctx *ctx, const gray_params *params, const gray_knobs *
 int gray_bvh_set_timing(gray_ctx *ctx, int enable);
 float gray_bvh_last_build_ms(gray_ctx *ctx);
 
+/* No reference counterpart (serving: allocation up front).  Allocates the
+ * collect-pass lists and task buffers for launches of up to max_lanes lanes
+ * (sum over the views of ceil(w/8)*ceil(h/8)*64) at list_capacity hits per
+ * ray, so that later gray_render_forward calls within those sizes allocate
+ * nothing.  Buffers never shrink. */
+int gray_render_reserve(gray_ctx *ctx, int64_t max_lanes, int32_t list_capacity);
+
 /* No reference counterpart (profiling aid).  Device milliseconds of the
  * stages of the last timed gray_render_forward (its first 8 views): ms[0] the
  * per-tile collect kernel, ms[1] the collect task rounds, ms[2] list
