@@ -98,17 +98,17 @@ __device__ __forceinline__ float ld<__half>(const __half *p, size_t i) {
 
 __device__ __forceinline__ int rest_count(int degree) { return (degree + 1) * (degree + 1) - 1; }
 
-template <typename T>
-__global__ void k_sh_forward(int n, int K, const int32_t *__restrict__ cur_degree, const uint8_t *__restrict__ enable,
-                             const float *__restrict__ mean, const T *__restrict__ dc, const T *__restrict__ rest,
-                             ShViews V) {
-    if (!*enable) return;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
-    const int degree = min(min(*cur_degree, 3), K >= 15 ? 3 : (K >= 8 ? 2 : (K >= 3 ? 1 : 0)));
-    const int kc = degree > 0 ? rest_count(degree) : 0;
-    float c[15][3];
-    for (int k = 0; k < kc; ++k)
+// Bodies are instantiated per degree so that every coefficient and basis
+// array has compile-time indices and lives in registers; the kernels switch
+// once on the (launch-uniform) degree read from the device.
+template <int DEG, typename T>
+__device__ __forceinline__ void sh_forward_body(int g, int K, const float *__restrict__ mean, const T *__restrict__ dc,
+                                                const T *__restrict__ rest, const ShViews &V) {
+    constexpr int KC = DEG > 0 ? (DEG + 1) * (DEG + 1) - 1 : 0;
+    float c[KC > 0 ? KC : 1][3];
+#pragma unroll
+    for (int k = 0; k < KC; ++k)
+#pragma unroll
         for (int ch = 0; ch < 3; ++ch) c[k][ch] = ld(rest, ((size_t)g * K + k) * 3 + ch);
     const float d0 = ld(dc, 3 * (size_t)g), d1 = ld(dc, 3 * (size_t)g + 1), d2 = ld(dc, 3 * (size_t)g + 2);
     const float mx = mean[3 * g], my = mean[3 * g + 1], mz = mean[3 * g + 2];
@@ -118,10 +118,11 @@ __global__ void k_sh_forward(int n, int K, const int32_t *__restrict__ cur_degre
         const float inv = 1.0f / sqrtf(fmaxf(dx * dx + dy * dy + dz * dz, 1e-20f));
         const float x = dx * inv, y = dy * inv, z = dz * inv;
         float r0 = kC0 * d0, r1 = kC0 * d1, r2 = kC0 * d2;
-        if (degree > 0) {
+        if constexpr (DEG > 0) {
             float b[15];
-            sh_basis(x, y, z, degree, b);
-            for (int k = 0; k < kc; ++k) {
+            sh_basis(x, y, z, DEG, b);
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
                 r0 = fmaf(b[k], c[k][0], r0);
                 r1 = fmaf(b[k], c[k][1], r1);
                 r2 = fmaf(b[k], c[k][2], r2);
@@ -134,25 +135,43 @@ __global__ void k_sh_forward(int n, int K, const int32_t *__restrict__ cur_degre
     }
 }
 
-// reference sh_backward_kernel (sh.cu:249-287), summed over the views
-__global__ void k_sh_backward(int n, int K, const int32_t *__restrict__ cur_degree, const uint8_t *__restrict__ enable,
-                              const float *__restrict__ mean, const float *__restrict__ rest,
-                              const uint8_t *__restrict__ visible, float *__restrict__ grad_dc,
-                              float *__restrict__ grad_rest, float *__restrict__ grad_mean, int accumulate,
-                              ShViews V) {
+__device__ __forceinline__ int sh_degree(const int32_t *cur_degree, int K) {
+    return min(min(*cur_degree, 3), K >= 15 ? 3 : (K >= 8 ? 2 : (K >= 3 ? 1 : 0)));
+}
+
+template <typename T>
+__global__ void k_sh_forward(int n, int K, const int32_t *__restrict__ cur_degree, const uint8_t *__restrict__ enable,
+                             const float *__restrict__ mean, const T *__restrict__ dc, const T *__restrict__ rest,
+                             ShViews V) {
     if (!*enable) return;
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
-    if (visible && !visible[g]) return;
-    const int degree = min(min(*cur_degree, 3), K >= 15 ? 3 : (K >= 8 ? 2 : (K >= 3 ? 1 : 0)));
-    const int kc = degree > 0 ? rest_count(degree) : 0;
-    float c[15][3];
-    for (int k = 0; k < kc; ++k)
+    switch (sh_degree(cur_degree, K)) {
+        case 3: sh_forward_body<3>(g, K, mean, dc, rest, V); break;
+        case 2: sh_forward_body<2>(g, K, mean, dc, rest, V); break;
+        case 1: sh_forward_body<1>(g, K, mean, dc, rest, V); break;
+        default: sh_forward_body<0>(g, K, mean, dc, rest, V); break;
+    }
+}
+
+// reference sh_backward_kernel (sh.cu:249-287), summed over the views
+template <int DEG>
+__device__ __forceinline__ void sh_backward_body(int g, int K, const float *__restrict__ mean,
+                                                 const float *__restrict__ rest, float *__restrict__ grad_dc,
+                                                 float *__restrict__ grad_rest, float *__restrict__ grad_mean,
+                                                 int accumulate, const ShViews &V) {
+    constexpr int KC = DEG > 0 ? (DEG + 1) * (DEG + 1) - 1 : 0;
+    constexpr int KA = KC > 0 ? KC : 1;
+    float c[KA][3];
+#pragma unroll
+    for (int k = 0; k < KC; ++k)
+#pragma unroll
         for (int ch = 0; ch < 3; ++ch) c[k][ch] = rest[((size_t)g * K + k) * 3 + ch];
     const float mx = mean[3 * g], my = mean[3 * g + 1], mz = mean[3 * g + 2];
     float gdc0 = 0.0f, gdc1 = 0.0f, gdc2 = 0.0f;
-    float grest[15][3];
-    for (int k = 0; k < kc; ++k) grest[k][0] = grest[k][1] = grest[k][2] = 0.0f;
+    float grest[KA][3];
+#pragma unroll
+    for (int k = 0; k < KC; ++k) grest[k][0] = grest[k][1] = grest[k][2] = 0.0f;
     float gm0 = 0.0f, gm1 = 0.0f, gm2 = 0.0f;
     for (int v = 0; v < V.num; ++v) {
         const float *o = V.origin[v];
@@ -176,11 +195,12 @@ __global__ void k_sh_backward(int n, int K, const int32_t *__restrict__ cur_degr
             gdc2 = kC0 * g2;
         }
         float gdx = 0.0f, gdy = 0.0f, gdz = 0.0f;
-        if (degree > 0) {
+        if constexpr (DEG > 0) {
             float b[15], bx[15], by[15], bz[15];
-            sh_basis(x, y, z, degree, b);
-            sh_basis_grad(x, y, z, degree, bx, by, bz);
-            for (int k = 0; k < kc; ++k) {
+            sh_basis(x, y, z, DEG, b);
+            sh_basis_grad(x, y, z, DEG, bx, by, bz);
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
                 if (accumulate) {
                     grest[k][0] += b[k] * g0;
                     grest[k][1] += b[k] * g1;
@@ -206,18 +226,39 @@ __global__ void k_sh_backward(int n, int K, const int32_t *__restrict__ cur_degr
         grad_dc[3 * g + 0] += gdc0;
         grad_dc[3 * g + 1] += gdc1;
         grad_dc[3 * g + 2] += gdc2;
-        for (int k = 0; k < kc; ++k)
+#pragma unroll
+        for (int k = 0; k < KC; ++k)
+#pragma unroll
             for (int ch = 0; ch < 3; ++ch) grad_rest[((size_t)g * K + k) * 3 + ch] += grest[k][ch];
     } else {
         grad_dc[3 * g + 0] = gdc0;
         grad_dc[3 * g + 1] = gdc1;
         grad_dc[3 * g + 2] = gdc2;
-        for (int k = 0; k < kc; ++k)
+#pragma unroll
+        for (int k = 0; k < KC; ++k)
+#pragma unroll
             for (int ch = 0; ch < 3; ++ch) grad_rest[((size_t)g * K + k) * 3 + ch] = grest[k][ch];
     }
     grad_mean[3 * g + 0] += gm0;
     grad_mean[3 * g + 1] += gm1;
     grad_mean[3 * g + 2] += gm2;
+}
+
+__global__ void k_sh_backward(int n, int K, const int32_t *__restrict__ cur_degree, const uint8_t *__restrict__ enable,
+                              const float *__restrict__ mean, const float *__restrict__ rest,
+                              const uint8_t *__restrict__ visible, float *__restrict__ grad_dc,
+                              float *__restrict__ grad_rest, float *__restrict__ grad_mean, int accumulate,
+                              ShViews V) {
+    if (!*enable) return;
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    if (visible && !visible[g]) return;
+    switch (sh_degree(cur_degree, K)) {
+        case 3: sh_backward_body<3>(g, K, mean, rest, grad_dc, grad_rest, grad_mean, accumulate, V); break;
+        case 2: sh_backward_body<2>(g, K, mean, rest, grad_dc, grad_rest, grad_mean, accumulate, V); break;
+        case 1: sh_backward_body<1>(g, K, mean, rest, grad_dc, grad_rest, grad_mean, accumulate, V); break;
+        default: sh_backward_body<0>(g, K, mean, rest, grad_dc, grad_rest, grad_mean, accumulate, V); break;
+    }
 }
 
 }  // namespace

@@ -124,7 +124,7 @@ typedef struct {
                                    compositing (0 = composite during traversal, re-traversing per batch).
                                    Rays with more hits fall back to re-traversal.  The context keeps
                                    list_capacity * 16 bytes per ray of one launch (<= 8 views),
-                                   plus 128 bytes per ray of continuation tasks. */
+                                   plus 256 bytes per ray of continuation tasks. */
     int32_t collect_budget;     /* iterations a ray's walk runs before its remaining subtrees become tasks
                                    that other lanes take (load balance of rays that cross many boxes);
                                    <= 0 = the built-in default */

@@ -317,3 +317,41 @@ def test_adam_matches_oracle():
         assert np.abs(getattr(g, "second_moment_" + name).cpu().numpy() - v).max() < 1e-6
         assert np.all(getattr(g, "grad_" + name).cpu().numpy() == 0)
     assert np.all(g.pruning_counter.cpu().numpy() == 1)
+
+
+def test_inference_only_fp16_sh_matches_oracle():
+    """inference_only keeps SH coefficients in fp16 (reference gaussians.h:132);
+    the image equals the oracle's on the same (fp16-rounded) coefficients."""
+    from gray_amd.native import NativeRaytracer
+
+    scene = scenes.tiny_scene(1024, seed=12)
+    cam = scenes.tiny_camera(48, 40)
+    nr = NativeRaytracer(48, 40, scene.count, 3, 300_000_000, 120_000_000, inference_only=True)
+    g = nr.get_gaussians()
+    assert g.sh_coeffs_rest.dtype == torch.float16
+    with torch.no_grad():
+        g.mean.copy_(scene.mean)
+        g.rotation.copy_(scene.rotation)
+        g.scale.copy_(scene.scale)
+        g.opacity.copy_(scene.opacity)
+        g.sh_coeffs_dc.copy_(scene.sh_dc)
+        g.sh_coeffs_rest.copy_(scene.sh_rest)
+        g.current_sh_degree.fill_(3)
+    from gpu_util import set_camera
+
+    set_camera(nr, cam)
+    nr.rebuild_bvh()
+    with torch.no_grad():
+        nr.forward_pass()
+    torch.cuda.synchronize()
+    nr.device_status()
+    dc16 = scene.sh_dc.half().float().numpy()
+    rest16 = scene.sh_rest.half().float().numpy()
+    rgb = O.sh_forward(scene.mean.numpy(), dc16, rest16, cam.origin.astype(np.float32), 3)
+    assert np.abs(g.channels.cpu().numpy() - rgb).max() < 1e-5
+    geo, _ = nr.debug_snapshot()
+    # no backward state in inference: the rays come from the closed form
+    ro, rd = O.pinhole_rays(48, 40, cam.fov_y, cam.origin.astype(np.float32), cam.rotation_c2w_blender())
+    ref = O.render(geo.cpu().numpy(), rgb, ro, rd, oracle_knobs(nr))
+    img = nr.get_framebuffer().output_channels[:40, :48].reshape(-1, 3).cpu().numpy()
+    assert np.abs(img - ref["C"]).max() < 1e-5
