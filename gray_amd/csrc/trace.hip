@@ -960,19 +960,26 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
         if (!alive) continue;
         kb.clear();
         int count = 0;
+        // software-pipelined scan: the next batch is in flight while this one
+        // is inserted
+        float2 nx[kListBatch];
+        auto fetch = [&](int i0) {  // unconditional loads (clamped index): exact wait counts
+#pragma unroll
+            for (int j = 0; j < kListBatch; ++j) {
+                const float4 rec = L[(size_t)min(i0 + j, n - 1) * S + lg];
+                nx[j] = i0 + j < n ? make_float2(rec.x, rec.y) : make_float2(INFINITY, __uint_as_float(0xFFFFFFFFu));
+            }
+        };
+        if (n > 0) fetch(0);
         for (int i0 = 0; i0 < n; i0 += kListBatch) {
             float d[kListBatch];
             uint32_t id[kListBatch];
 #pragma unroll
             for (int j = 0; j < kListBatch; ++j) {
-                d[j] = INFINITY;
-                id[j] = 0xFFFFFFFFu;
-                if (i0 + j < n) {
-                    const float4 rec = L[(size_t)(i0 + j) * S + lg];
-                    d[j] = rec.x;
-                    id[j] = __float_as_uint(rec.y) & ~kListMark;
-                }
+                d[j] = nx[j].x;
+                id[j] = __float_as_uint(nx[j].y) & ~kListMark;
             }
+            if (i0 + kListBatch < n) fetch(i0 + kListBatch);
 #pragma unroll
             for (int j = 0; j < kListBatch; ++j) {
                 const bool beyond = strict ? d[j] > cur_d : (d[j] > cur_d || (d[j] == cur_d && id[j] > cur_id));
@@ -1031,40 +1038,41 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
         }
     }
 
-    // skip set: every entry compositing did not flag
+    // skip set: every entry compositing did not flag.  Pipelined like the
+    // scan: the colours of this batch are gathered (unconditionally; flagged
+    // and padding entries get weight 0, which leaves the sums bit for bit as
+    // if skipped) while the next batch's ids and alphas are in flight.
     float skc0 = 0.0f, skc1 = 0.0f, skc2 = 0.0f, ska = 0.0f, pnc = 1.0f;
     const int nskip = traced ? n : 0;
+    float2 sx[kListBatch];
+    auto fetch_skip = [&](int i0) {
+#pragma unroll
+        for (int j = 0; j < kListBatch; ++j) {
+            const float4 rec = L[(size_t)min(i0 + j, nskip - 1) * S + lg];
+            sx[j] = make_float2(rec.y, i0 + j < nskip ? rec.z : 0.0f);
+        }
+    };
+    if (nskip > 0) fetch_skip(0);
     for (int i0 = 0; i0 < nskip; i0 += kListBatch) {
-        uint32_t id[kListBatch];
-        float a[kListBatch];
+        float w[kListBatch], c[kListBatch][3];
 #pragma unroll
         for (int j = 0; j < kListBatch; ++j) {
-            id[j] = kListMark;
-            a[j] = 0.0f;
-            if (i0 + j < nskip) {
-                const float4 rec = L[(size_t)(i0 + j) * S + lg];
-                id[j] = __float_as_uint(rec.y);
-                a[j] = rec.z;
-            }
+            const uint32_t id = __float_as_uint(sx[j].x);
+            w[j] = (id & kListMark) ? 0.0f : sx[j].y;
+            const uint32_t gid = id & ~kListMark;
+            c[j][0] = rgb[3 * gid + 0];
+            c[j][1] = rgb[3 * gid + 1];
+            c[j][2] = rgb[3 * gid + 2];
         }
-        float c[kListBatch][3];
+        if (i0 + kListBatch < nskip) fetch_skip(i0 + kListBatch);
 #pragma unroll
         for (int j = 0; j < kListBatch; ++j) {
-            c[j][0] = c[j][1] = c[j][2] = 0.0f;
-            if (!(id[j] & kListMark)) {
-                c[j][0] = rgb[3 * id[j] + 0];
-                c[j][1] = rgb[3 * id[j] + 1];
-                c[j][2] = rgb[3 * id[j] + 2];
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < kListBatch; ++j) {
-            if (id[j] & kListMark) continue;
-            skc0 += a[j] * c[j][0];
-            skc1 += a[j] * c[j][1];
-            skc2 += a[j] * c[j][2];
-            ska += a[j];
-            pnc = pnc * (1.0f - a[j]);
+            if (w[j] == 0.0f) continue;
+            skc0 += w[j] * c[j][0];
+            skc1 += w[j] * c[j][1];
+            skc2 += w[j] * c[j][2];
+            ska += w[j];
+            pnc = pnc * (1.0f - w[j]);
         }
     }
     if (A.tile_stats && lane == 0) A.tile_stats[4 * (size_t)(A.tile_base_launch + tile) + 2] = nrounds;
