@@ -25,7 +25,7 @@
 #define GRAY_LEAF 0x80000000u   // child reference flag: leaf, payload = Gaussian id
 #define GRAY_EMPTY 0xFFFFFFFFu  // child reference: nothing
 
-// Per-Gaussian geometry snapshot (64 B, read with one scalar load per leaf).
+// Per-Gaussian geometry snapshot (64 B: four 16-byte loads per leaf test).
 // m = world-to-local linear part W2L = diag(1/(s*clip*gsf)) * R^T, row-major;
 // op = sigmoid(opacity) (or -1 when invisible), clip = adaptive clamp factor
 // (reference kernel.cu:3-22).
@@ -37,7 +37,7 @@ struct __attribute__((aligned(16))) GeoRecord {
     float pad0, pad1;
 };
 
-// BVH2 internal node with both child boxes inline (64 B, one scalar load).
+// BVH2 internal node with both child boxes inline (64 B: four 16-byte loads).
 struct __attribute__((aligned(16))) BvhNode {
     float lb[6];  // left child box: min xyz, max xyz
     float rb[6];  // right child box

@@ -1,23 +1,30 @@
-// Wavefront-coherent Gaussian ray tracing on gfx950: forward, backward, debug.
+// Gaussian ray tracing on gfx950: forward, backward, debug.
 //
-// One workgroup = one wave64 = one 8x8 tile of rays of one view.  The wave
-// walks the LBVH as a packet: node and Gaussian records are fetched once per
-// wave with scalar loads (their addresses are wave-uniform), every lane runs
-// the slab / exact ellipsoid test for its own ray, and a ballot decides the
-// descent.  Every lane keeps its GRAY_KB nearest hits (distance, id, alpha)
-// sorted in LDS; compositing follows the reference batch semantics
-// (forward_pass.cu:150-230): each batch takes the 32 nearest hits strictly
-// beyond tmin (served in sub-rounds of GRAY_KB), composites front to back and
-// stops at T < t_threshold.
+// Rays are grouped in 8x8 tiles of one view; a wave64 takes a tile, every lane
+// one ray.  The forward runs in passes:
+//   collect    every lane walks the LBVH for its own ray and appends every hit
+//              (distance, id, alpha) to the ray's list; walks that run long
+//              hand their remaining subtrees on as tasks, finished by
+//              persistent task rounds (load balance of grazing rays);
+//   composite  per ray, the reference batch semantics over the list
+//              (forward_pass.cu:150-230): each batch takes the 32 nearest hits
+//              strictly beyond tmin (served in sub-rounds of GRAY_KB from a
+//              register k-buffer), composites front to back and stops at
+//              T < t_threshold;
+//   retraverse the same forward with the BVH as the hit source, for rays whose
+//              list overflowed (and for every ray when lists are off).
+// The backward replays each ray's stored composited list back to front
+// (re-traversing rays whose list exceeded the store) and sums the gradients
+// of a tile's rays per Gaussian in LDS before the global atomics.
 //
 // Truncation bookkeeping.  The reference estimates the colour of the hits it
 // never composited as (sum_all a*c - sum_comp a*c) / (sum_all a - sum_comp a)
 // and the full transmittance as prod_all (1-a) (forward_pass.cu:53-57,
 // 206-208, 238-240; shaders.cu:54-57).  The same quantities are formed here
-// directly over the non-composited set ("skip set"): a hit enters it when the
-// k-buffer rejects or evicts it, or when compositing stops before it; later
-// rounds take composited hits back out.  F = T * prod_skip (1-a).  This is the
-// same mathematics without the fp32 cancellation of sum_all - sum_comp.
+// directly over the non-composited set ("skip set"): the hits the k-buffer
+// never handed to compositing, hits beyond zfar and hits after termination.
+// F = T * prod_skip (1-a).  This is the same mathematics without the fp32
+// cancellation of sum_all - sum_comp.
 
 #include <float.h>
 #include <math.h>
@@ -105,11 +112,11 @@ __device__ __forceinline__ bool make_ray(const DevView &V, int x, int y, LaneRay
     return l2 != 0.0f;
 }
 
-// Wave-uniform node and Gaussian records are fetched with broadcast vector
-// loads (every lane reads the same 64 bytes; the memory system serves one line
-// per wave).  Vector loads complete in order (vmcnt), so a prefetched record
-// does not force a wait on the LDS traffic of the k-buffer the way an
-// out-of-order scalar load (lgkmcnt) would.
+// Node and Gaussian records are fetched with 16-byte vector loads (four per
+// 64-byte record).  Vector loads complete in order (vmcnt), so a prefetched
+// record does not force a wait on LDS traffic the way an out-of-order scalar
+// load (lgkmcnt) would; in_vgpr keeps an address that happens to be
+// wave-uniform in a vector register for that reason.
 struct NodeV {
     float lb[6], rb[6];
     uint32_t left, right;
@@ -149,10 +156,6 @@ __device__ __forceinline__ void load_rgb(const float *__restrict__ rgb, uint32_t
     c[0] = q[0];
     c[1] = q[1];
     c[2] = q[2];
-}
-
-__device__ __forceinline__ float uniform_f(float v) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
 // Per-ray traversal.  Every lane walks the LBVH for its own ray with its own
