@@ -68,3 +68,20 @@ def test_capacities_hold_the_cap():
         lanes = min(views, 8) * tiles * 64
         used = 8 * stored * views * w * h + 16 * lists * lanes + 256 * lanes
         assert used <= 4 << 30 and lists > 0
+
+
+def test_expon_lr_schedule_closed_forms():
+    import math
+
+    from gray_amd.training import expon_lr
+
+    f = expon_lr(1e-2, 1e-4, max_steps=1000)
+    assert math.isclose(f(0), 1e-2, rel_tol=1e-12)
+    assert math.isclose(f(1000), 1e-4, rel_tol=1e-12)
+    assert math.isclose(f(5000), 1e-4, rel_tol=1e-12)  # constant after max_steps
+    assert math.isclose(f(500), math.sqrt(1e-2 * 1e-4), rel_tol=1e-12)  # geometric midpoint
+    assert all(f(s) > f(s + 1) for s in range(0, 1000, 37))
+    assert f(-1) == 0.0 and expon_lr(0.0, 0.0)(10) == 0.0
+    d = expon_lr(1e-2, 1e-4, delay_steps=100, delay_mult=0.01, max_steps=1000)
+    assert math.isclose(d(0), 0.01 * 1e-2, rel_tol=1e-12)  # eased in from delay_mult * lr
+    assert math.isclose(d(100), f(100), rel_tol=1e-12)  # ramp complete
