@@ -82,6 +82,15 @@ class Adam(C.Structure):
     ]
 
 
+class LossArgs(C.Structure):
+    _fields_ = [
+        ("planes", i32), ("height", i32), ("width", i32), ("valid_padding", i32), ("train", i32), ("pad", i32),
+        ("render", vp), ("render_stride", i64 * 3), ("target", vp), ("target_stride", i64 * 3),
+        ("grad", vp), ("grad_stride", i64 * 3), ("lambda_ssim", C.c_float), ("grad_scale", C.c_float),
+        ("grad_scale_device", vp), ("workspace", vp), ("loss", vp),
+    ]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
@@ -111,6 +120,9 @@ def _load():
         "gray_debug_list_counts": (i32, [vp, vp, i64, C.POINTER(i64), vp]),
         "gray_render_stage_ms": (i32, [vp, C.POINTER(C.c_float), i32]),
         "gray_render_reserve": (i32, [vp, i64, i32]),
+        "gray_loss_workspace_bytes": (i32, [i32, i32, i32, C.POINTER(i64)]),
+        "gray_loss_forward": (i32, [C.POINTER(LossArgs), vp]),
+        "gray_loss_backward": (i32, [C.POINTER(LossArgs), vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -119,7 +131,7 @@ def _load():
     return lib
 
 
-ABI_VERSION = 3  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
+ABI_VERSION = 4  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
 LIB = _load()
 if LIB.gray_abi_version() != ABI_VERSION:
     raise ImportError(f"libgray_hip.so has ABI {LIB.gray_abi_version()}, the binding expects {ABI_VERSION}; "
@@ -129,7 +141,7 @@ EXPORTED = (
     "gray_bvh_update", "gray_bvh_set_timing", "gray_bvh_last_build_ms", "gray_bvh_info", "gray_sh_forward",
     "gray_sh_backward", "gray_render_forward", "gray_render_backward", "gray_adam_step", "gray_debug_hitlists",
     "gray_debug_snapshot", "gray_debug_bvh", "gray_debug_list_counts", "gray_render_stage_ms",
-    "gray_render_reserve",
+    "gray_render_reserve", "gray_loss_workspace_bytes", "gray_loss_forward", "gray_loss_backward",
 )
 
 

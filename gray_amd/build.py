@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB_PATH = os.path.join(HERE, "libgray_hip.so")
-SOURCES = ["capi.hip", "bvh.hip", "trace.hip", "sh.hip", "adam.hip"]
+SOURCES = ["capi.hip", "bvh.hip", "trace.hip", "sh.hip", "adam.hip", "loss.hip"]
 ARCH = os.environ.get("GRAY_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 instructions are disabled for the device pass (v_pk_* fp32 forms

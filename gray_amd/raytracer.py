@@ -73,6 +73,7 @@ class Raytracer(torch.nn.Module):
 
         # last output, kept for the backward pass
         self.output_channels = None
+        self._loss = None  # FramebufferLoss of backward_l1_ssim
 
     def __call__(self, cam_info: CameraInfo, znear=0.0, zfar=99999.9, skip_copy=False):
         "Render the scene from cam_info; returns [C, h, w]."
@@ -109,6 +110,25 @@ class Raytracer(torch.nn.Module):
             framebuffer.grad_output_channels[:h, :w].copy_(self.output_channels.grad.moveaxis(0, -1))
             self.output_channels = None
         self.cuda_module.backward_pass()
+
+    def backward_l1_ssim(self, target: torch.Tensor, lambda_ssim: float = 0.2, scale: float = 1.0) -> torch.Tensor:
+        """The reference training step's loss and backward (train.py:255-262)
+        with the fused loss kernels: `scale` * gradient of (1 - lambda) *
+        l1(render, target) + lambda * (1 - SSIM(render, target)) goes straight
+        into the framebuffer's grad_output_channels (no autograd hop and no
+        copy, gray/raytracer.py:175-179), then the ray tracer's backward runs.
+        target [C, h, w]; returns the device tensor {loss, l1, ssim} (unscaled)."""
+        assert self.output_channels is not None, "backward_l1_ssim needs a forward pass with gradients enabled"
+        if self._loss is None:
+            from .loss import FramebufferLoss
+
+            self._loss = FramebufferLoss(self.cuda_module.device)
+        fb = self.cuda_module.get_framebuffer()
+        loss = self._loss(fb.output_channels, fb.grad_output_channels, self.render_height, self.render_width,
+                          target, lambda_ssim, scale)
+        self.output_channels = None
+        self.cuda_module.backward_pass()
+        return loss
 
     def step(self):
         # pruning counter update (update_pruning_stats) is fused into the Adam launch

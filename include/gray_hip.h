@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 3
+#define GRAY_ABI_VERSION 4
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -164,6 +164,25 @@ typedef struct {
     const int32_t *sh_update_laziness; /* [1] or NULL */
 } gray_adam;
 
+/* Training loss over P strided fp32 planes of H x W pixels (render and
+ * target), e.g. a [C,H,W] image (strides {H*W, W, 1}) or the [H,W,C]
+ * framebuffer (strides {1, W_alloc*C, C}). */
+typedef struct {
+    int32_t planes, height, width;
+    int32_t valid_padding;     /* 0: SSIM over every pixel, zero padding ("same");
+                                  1: only pixels >= 5 from the border ("valid") */
+    int32_t train;             /* 1: the forward also writes the SSIM derivative maps the backward reads */
+    int32_t pad;
+    const float *render;  int64_t render_stride[3];  /* plane, row, column (elements) */
+    const float *target;  int64_t target_stride[3];
+    float *grad;          int64_t grad_stride[3];    /* backward output, may alias nothing above */
+    float lambda_ssim;         /* loss = (1 - lambda) mean|render - target| + lambda (1 - mean SSIM) */
+    float grad_scale;          /* grad = grad_scale * (*grad_scale_device) * dloss/drender */
+    const float *grad_scale_device; /* device [1] or NULL (= 1) */
+    void *workspace;           /* device, gray_loss_workspace_bytes(planes, height, width) bytes */
+    float *loss;               /* device [3]: loss, mean |render - target|, mean SSIM */
+} gray_loss_args;
+
 /* ---- context -------------------------------------------------------------- */
 
 /* Replaces the torch custom-class ctor gray.Raytracer (cuda/raytracer.h:49-86).
@@ -261,6 +280,24 @@ int gray_render_backward(gray_ctx *ctx, const gray_render_args *args, const gray
  * bit1 zero_grads, bit2 update SH (enable_sh && step % sh_update_laziness == 0);
  * each is replaced by the device-resident switch in gray_adam when given. */
 int gray_adam_step(gray_ctx *ctx, const gray_adam *adam, int32_t step, int32_t flags, void *stream);
+
+/* ---- training loss ----------------------------------------------------- */
+
+/* Replaces the loss of train.py:255-260 (F.l1_loss + the third-party
+ * fused_ssim, git a7c48d6) and the gradient copy into the framebuffer of
+ * gray/raytracer.py:175-179.  SSIM: 11-tap Gaussian window (sigma 1.5),
+ * C1 = 0.01^2, C2 = 0.03^2.  Workspace needed by the two calls below. */
+int gray_loss_workspace_bytes(int32_t planes, int32_t height, int32_t width, int64_t *bytes);
+
+/* Writes args->loss (deterministic: fixed-order double reduction) and, when
+ * args->train, the derivative maps into the workspace. */
+int gray_loss_forward(const gray_loss_args *args, void *stream);
+
+/* Writes args->grad from the maps of the preceding gray_loss_forward (same
+ * args, train = 1, same workspace, not reused in between).  The L1 term's
+ * gradient is sign(render - target) / (P H W), sign(0) = 0, as torch's
+ * l1_loss. */
+int gray_loss_backward(const gray_loss_args *args, void *stream);
 
 /* ---- debugging / parity --------------------------------------------------- */
 

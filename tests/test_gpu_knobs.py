@@ -1,0 +1,64 @@
+"""Forward and backward knobs of the reference config on the GPU against the
+oracle: kernel exponent p (exp_power 1, 2, 3), alpha threshold, background,
+depth output (render_depth, depth normalised by 1 - T), and the znear/zfar
+interval, in the collected-lists and the re-traversal forward."""
+import numpy as np
+import pytest
+import torch
+
+from gray_amd import scenes
+from oracle import oracle as O
+
+from gpu_util import rays_of, rel_err, setup_native
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    dict(exp_power=1.0, alpha_threshold=0.01, bg=(0.0, 0.0, 0.0)),
+    dict(exp_power=3.0, alpha_threshold=0.05, bg=(0.2, 0.5, 0.9)),
+    dict(exp_power=2.0, alpha_threshold=0.2, bg=(1.0, 1.0, 1.0)),
+    dict(exp_power=2.0, alpha_threshold=0.01, bg=(0.1, 0.1, 0.1), znear=2.6, zfar=3.4),
+]
+
+
+@pytest.mark.parametrize("lists", [None, 0], ids=["lists", "traverse"])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_knobs_forward_backward(case, lists):
+    kw = dict(CASES[case])
+    znear, zfar = kw.pop("znear", 0.0), kw.pop("zfar", 99999.9)
+    scene = scenes.tiny_scene(1024, seed=20 + case)
+    cam = scenes.tiny_camera(40, 32)
+    nr = setup_native(scene, cam, t_threshold=0.02, znear=znear, zfar=zfar, list_capacity=lists, **kw)
+    nr.get_config().render_depth.fill_(True)
+    torch.set_grad_enabled(True)
+    nr.forward_pass()
+    fb = nr.get_framebuffer()
+    fb.grad_output_channels.copy_(torch.randn(fb.grad_output_channels.shape,
+                                              generator=torch.Generator().manual_seed(case)).cuda())
+    g = nr.get_gaussians()
+    for name in ("grad_mean", "grad_rotation", "grad_scale", "grad_opacity", "grad_channels"):
+        getattr(g, name).zero_()
+    nr.backward_pass()
+    torch.cuda.synchronize()
+    nr.device_status()
+    geo, _ = nr.debug_snapshot()
+    geo = geo.cpu().numpy()
+    ro, rd = rays_of(nr)
+    rgb = g.channels.cpu().numpy()
+    k = O.make_knobs(t_threshold=0.02, znear=znear, zfar=zfar, render_depth=True, **kw)
+    ref = O.render(geo, rgb, ro, rd, k)
+    img = fb.output_channels[:32, :40].reshape(-1, 3).cpu().numpy()
+    depth = fb.output_depth[:32, :40].reshape(-1).cpu().numpy()
+    assert np.abs(img - ref["C"]).max() < 1e-5
+    assert np.abs(depth - ref["D"]).max() < 1e-4 * max(1.0, np.abs(ref["D"]).max())
+    if znear > 0.0:
+        k_open = O.make_knobs(t_threshold=0.02, render_depth=True, **kw)
+        assert ref["ncomp"].sum() < O.render(geo, rgb, ro, rd, k_open)["ncomp"].sum(), "the interval must cut hits"
+        assert ref["D"].max() <= zfar
+    _, l2w = O.snapshot(scene.mean, scene.rotation, scene.scale, scene.opacity, kw["alpha_threshold"],
+                        kw["exp_power"])
+    gout = fb.grad_output_channels[:32, :40].reshape(-1, 3).cpu().numpy()
+    ob = O.backward(geo, l2w, rgb, scene.rotation, scene.scale, scene.opacity, ro, rd, ref, gout, k, 40 * 32)
+    for name, key in (("grad_opacity", "opacity"), ("grad_scale", "scale"), ("grad_rotation", "rotation"),
+                      ("grad_channels", "rgb")):
+        assert rel_err(getattr(g, name).cpu(), ob[key]) < 1e-3, name

@@ -1,0 +1,123 @@
+"""Fused L1 + SSIM loss kernels on the GPU against the float64 loss oracle.
+
+Bars: loss within 1e-5 of the oracle (absolute, the loss is O(0.1-1));
+gradient within 1e-4 of the oracle's largest component (fp32 windowed sums
+against fp64)."""
+import numpy as np
+import pytest
+import torch
+
+from gray_amd import loss as FL
+from gray_amd import scenes
+from oracle import loss_ref as R
+
+from gpu_util import rel_err, setup_native
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(shape, seed):
+    g = torch.Generator().manual_seed(seed)
+    x = torch.rand(shape, generator=g) * 1.2 - 0.1          # unclamped render
+    y = (x + 0.15 * torch.randn(shape, generator=g)).clamp(0, 1)
+    return x, y
+
+
+@pytest.mark.parametrize("shape,lam,padding", [
+    ((3, 37, 23), 0.2, "same"),      # ragged: partial tiles on both axes
+    ((3, 545, 980), 0.2, "same"),    # the training camera
+    ((3, 64, 48), 0.0, "same"),      # L1 only
+    ((3, 64, 48), 1.0, "valid"),     # SSIM only, border excluded
+    ((2, 3, 40, 33), 0.2, "valid"),  # batch
+    ((1, 1, 1), 0.2, "same"),        # one pixel
+])
+def test_loss_and_gradient_match_oracle(shape, lam, padding):
+    x, y = pair(shape, seed=len(shape) * 1000 + shape[-1])
+    ref_loss, ref_grad = R.l1_ssim_grad(x, y, lam, padding)
+    xc = x.cuda().requires_grad_()
+    out = FL.l1_ssim_loss(xc, y.cuda(), lam, padding)
+    out.backward(torch.tensor(0.5, device="cuda"))
+    torch.cuda.synchronize()
+    assert abs(float(out.detach()) - ref_loss) < 1e-5
+    assert rel_err(xc.grad.cpu(), 0.5 * ref_grad) < 1e-4
+
+
+def test_fused_ssim_signature_matches_oracle():
+    x, y = pair((2, 3, 50, 70), seed=7)
+    ref = R.ssim(x.double(), y.double())
+    xd = x.double().requires_grad_()
+    R.ssim(xd, y.double()).backward()
+    xc = x.cuda().requires_grad_()
+    s = FL.fused_ssim(xc, y.cuda())
+    s.backward()
+    assert abs(float(s) - float(ref)) < 1e-5
+    assert rel_err(xc.grad.cpu(), xd.grad) < 1e-4
+    with torch.no_grad():
+        s_eval = FL.fused_ssim(x.cuda(), y.cuda(), train=False)
+    assert float(s_eval) == float(s)
+
+
+def test_strided_layouts_are_bit_identical():
+    x, y = pair((3, 45, 61), seed=3)
+    hwc = x.permute(1, 2, 0).contiguous().cuda()  # [H,W,C] like the framebuffer
+    a = hwc.permute(2, 0, 1).requires_grad_()     # non-contiguous [C,H,W] view
+    b = x.cuda().requires_grad_()
+    la, lb = FL.l1_ssim_loss(a, y.cuda()), FL.l1_ssim_loss(b, y.cuda())
+    la.backward()
+    lb.backward()
+    assert float(la) == float(lb)
+    assert torch.equal(a.grad, b.grad)
+
+
+def test_deterministic():
+    x, y = pair((3, 200, 300), seed=9)
+    xs = [x.cuda().requires_grad_() for _ in range(2)]
+    ls = [FL.l1_ssim_loss(t, y.cuda()) for t in xs]
+    for l_ in ls:
+        l_.backward()
+    assert float(ls[0]) == float(ls[1]) and torch.equal(xs[0].grad, xs[1].grad)
+
+
+def test_raytracer_fused_backward_equals_autograd_path():
+    """Raytracer.backward_l1_ssim against the reference's sequence: autograd
+    loss on the [C,H,W] render, raytracer.backward (copy into the
+    framebuffer, backward pass)."""
+    from gray_amd import Raytracer, RaytracerConfig
+
+    scene = scenes.tiny_scene(2048, seed=5)
+    cam = scenes.tiny_camera(72, 56)
+    target = torch.rand(3, 56, 72, generator=torch.Generator().manual_seed(2)).cuda()
+    grads = []
+    for fused in (False, True):
+        rt = Raytracer(RaytracerConfig(), scene.count, 72, 56)
+        g = rt.cuda_module.get_gaussians()
+        with torch.no_grad():
+            for name in ("mean", "rotation", "scale", "opacity"):
+                getattr(g, name).copy_(getattr(scene, name))
+            g.sh_coeffs_dc.copy_(scene.sh_dc)
+            g.sh_coeffs_rest.copy_(scene.sh_rest)
+            g.current_sh_degree.fill_(3)
+        rt.cuda_module.rebuild_bvh()
+        torch.set_grad_enabled(True)
+        render = rt(cam)
+        if fused:
+            out = rt.backward_l1_ssim(target, 0.2, scale=0.5)
+            loss = float(out[0])
+        else:
+            loss_t = FL.l1_ssim_loss(render, target, 0.2)
+            loss = float(loss_t)
+            rt.backward(loss_t * 0.5)
+        torch.cuda.synchronize()
+        rt.cuda_module.device_status()
+        fb = rt.cuda_module.get_framebuffer()
+        grads.append((loss, fb.grad_output_channels[:56, :72].clone(),
+                      {n: getattr(g, n).clone() for n in ("grad_mean", "grad_opacity", "grad_scale",
+                                                            "grad_sh_coeffs_dc")}))
+    (l0, go0, p0), (l1, go1, p1) = grads
+    assert abs(l0 - l1) < 1e-6
+    ref_loss, ref_grad = R.l1_ssim_grad(render.detach().cpu(), target.cpu(), 0.2)
+    assert abs(l1 - ref_loss) < 1e-5
+    assert rel_err(go1.permute(2, 0, 1).cpu(), 0.5 * ref_grad) < 1e-4
+    assert torch.equal(go0, go1)
+    for n in p0:
+        assert rel_err(p1[n].cpu(), p0[n].cpu()) < 1e-4, n
