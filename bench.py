@@ -78,16 +78,25 @@ def setup_raytracer(scene, cam, cfg):
     return rt
 
 
-def make_step(rt, cam, target):
+def make_step(rt, cam, target, loss="l1"):
+    """One training step.  "l1": the reference sequence with an L1 loss
+    (render, F.l1_loss, autograd, copy into the framebuffer, backward, Adam,
+    refit).  "l1ssim": the reference's default loss (lambda_ssim 0.2,
+    train.py:255-260) through the fused loss kernels."""
     import torch
 
     def step():
         render = rt(cam)
-        loss = torch.nn.functional.l1_loss(render, target)
-        rt.backward(loss)
+        loss_t = torch.nn.functional.l1_loss(render, target)
+        rt.backward(loss_t)
         rt.step()
 
-    return step
+    def step_l1ssim():
+        rt(cam)
+        rt.backward_l1_ssim(target, 0.2)
+        rt.step()
+
+    return step_l1ssim if loss == "l1ssim" else step
 
 
 def cpu_baseline(scene, cam, cfg, sample_rays: int, seed: int = 0):
@@ -301,6 +310,12 @@ def run_train(args, traffic):
                           "frac": round(step_alg / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                           "measured_copy_GBs": round(copy_peak_gbs(), 1)},
     }
+    if args.loss_steps > 0:
+        log(f"[bench] {args.loss_steps} steps with the L1 + SSIM loss (lambda 0.2, fused loss kernels)")
+        step_ls = make_step(rt, cam, target, "l1ssim")
+        step_ls()
+        ls_ms, _ = _median_ms(step_ls, args.loss_steps)
+        out["step_l1_ssim_ms_p50"] = round(ls_ms, 4)
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline on {args.cpu_rays} rays")
         out["cpu_baseline"] = cpu_baseline(scene, cam, cfg, args.cpu_rays)
@@ -513,6 +528,7 @@ def main():
     ap.add_argument("--render-list-gb", type=float, default=48.0, help="render: hit-list memory of a launch")
     ap.add_argument("--cpu-rays", type=int, default=32768)
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--loss-steps", type=int, default=20, help="train: further steps with the L1 + SSIM loss")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)

@@ -19,8 +19,9 @@
 //     (1/N) sum_p G(p - q) [A(p) + 2 x(q) B(p) + y(q) C(p)]
 // with A = dm/dmu1 (E held), B = dm/dE11, C = dm/dE12.
 //
-// Pass 1 (k_loss_forward): one 16x16 tile of one plane per workgroup; the
-// tile and its 5-pixel halo of x and y are staged in LDS, the five windowed
+// Pass 1 (k_loss_forward): one 16x16 tile of up to three planes (the channels
+// of an image) per workgroup; the tile and its 5-pixel halo of x and y are
+// staged in LDS with every load in flight at once, the five windowed
 // moments are formed by a horizontal then a vertical 11-tap pass, and the
 // workgroup writes its partial sums of |x - y| and m plus (when training) the
 // maps A, B, C.  Pass 2 (k_loss_backward): the same tiling over the three
@@ -39,6 +40,7 @@ constexpr int kRadius = 5;                // window radius (11 taps)
 constexpr int kTaps = 2 * kRadius + 1;
 constexpr int kSpan = kTile + 2 * kRadius;  // 26: tile plus halo
 constexpr int kThreads = kTile * kTile;   // 256, one output pixel per thread
+constexpr int kGroup = 3;                 // planes per workgroup (the C = 3 of an image)
 constexpr float kC1 = 0.01f * 0.01f;
 constexpr float kC2 = 0.03f * 0.03f;
 
@@ -46,13 +48,13 @@ struct LossArgs {
     const float *x, *y;
     float *grad;
     float *maps;       // [3][P][H][W]: A, B, C
-    float *partials;   // [P * tiles][2]: sum |x - y|, sum m
+    float *partials;   // [groups * tiles][2]: sum |x - y|, sum m
     float *loss;       // {loss, mean |x - y|, mean SSIM}
     const float *scale_dev;
     int64_t xs[3], ys[3], gs[3];  // plane, row, column strides (elements)
     int64_t plane_px;  // H * W
     int32_t P, H, W, valid, train;
-    int32_t tiles_x, tiles, xcd_chunk;
+    int32_t tiles_x, tiles, xcd_chunk, groups;
     float win[kTaps];
     float lambda, scale_host;
     float w_l1, w_ssim;  // gradient weights: (1 - lambda) / N_l1, -lambda / N_ssim
@@ -78,88 +80,129 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// Stages the span (tile + halo) of up to kGroup planes of `src` into lds.
+// Every lane issues all of its loads before the first LDS write: addresses are
+// clamped into the image (and plane indices into [0, P)), values outside the
+// image are replaced by zero afterwards, so no load is conditional.
+template <int kMaps>
+__device__ __forceinline__ void stage_span(float (*lds)[kSpan][kSpan + 1], const float *const *src,
+                                           const int64_t *plane_stride, const int64_t *row_stride,
+                                           const int64_t *col_stride, int g0, int P, int W, int H, int x0, int y0,
+                                           int tid) {
+    constexpr int kIters = (kSpan * kSpan + kThreads - 1) / kThreads;  // 3
+    float v[kIters][kMaps][kGroup];
+    bool in[kIters];
+#pragma unroll
+    for (int j = 0; j < kIters; ++j) {
+        const int i = tid + j * kThreads;
+        const int ii = i < kSpan * kSpan ? i : 0;
+        const int r = ii / kSpan, c = ii - r * kSpan;
+        const int gx = x0 + c, gy = y0 + r;
+        in[j] = i < kSpan * kSpan && gx >= 0 && gy >= 0 && gx < W && gy < H;
+        const int cx = min(max(gx, 0), W - 1), cy = min(max(gy, 0), H - 1);
+#pragma unroll
+        for (int m = 0; m < kMaps; ++m)
+#pragma unroll
+            for (int q = 0; q < kGroup; ++q) {
+                const int pq = min(g0 + q, P - 1);
+                v[j][m][q] = src[m][pq * plane_stride[m] + cy * row_stride[m] + cx * col_stride[m]];
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < kIters; ++j) {
+        const int i = tid + j * kThreads;
+        if (i < kSpan * kSpan) {
+            const int r = i / kSpan, c = i - r * kSpan;
+#pragma unroll
+            for (int m = 0; m < kMaps; ++m)
+#pragma unroll
+                for (int q = 0; q < kGroup; ++q) lds[m * kGroup + q][r][c] = in[j] ? v[j][m][q] : 0.0f;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_loss_forward(LossArgs A) {
-    __shared__ float sx[kSpan][kSpan + 1];
-    __shared__ float sy[kSpan][kSpan + 1];
+    __shared__ float sxy[2 * kGroup][kSpan][kSpan + 1];  // x of each plane, then y of each plane
     __shared__ float hs[5][kSpan][kTile + 1];
     __shared__ float red[2][kThreads / 64];
 
     const int tile = tile_of(A);
     if (tile >= A.tiles) return;
-    const int p = blockIdx.y;
+    const int g0 = blockIdx.y * kGroup;
+    const int np = min(kGroup, A.P - g0);
     const int tid = threadIdx.x;
     const int bx = tile % A.tiles_x, by = tile / A.tiles_x;
     const int x0 = bx * kTile - kRadius, y0 = by * kTile - kRadius;
-    const float *__restrict__ X = A.x + (int64_t)p * A.xs[0];
-    const float *__restrict__ Y = A.y + (int64_t)p * A.ys[0];
-
-    for (int i = tid; i < kSpan * kSpan; i += kThreads) {
-        const int r = i / kSpan, c = i - r * kSpan;
-        const int gx = x0 + c, gy = y0 + r;
-        const bool in = gx >= 0 && gy >= 0 && gx < A.W && gy < A.H;
-        sx[r][c] = in ? X[gy * A.xs[1] + gx * A.xs[2]] : 0.0f;
-        sy[r][c] = in ? Y[gy * A.ys[1] + gx * A.ys[2]] : 0.0f;
-    }
-    __syncthreads();
-
-    // horizontal pass: rows of the span, columns of the tile
-    for (int i = tid; i < kSpan * kTile; i += kThreads) {
-        const int r = i / kTile, c = i - r * kTile;
-        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;
-#pragma unroll
-        for (int k = 0; k < kTaps; ++k) {
-            const float xv = sx[r][c + k], yv = sy[r][c + k], g = A.win[k];
-            a0 += g * xv;
-            a1 += g * yv;
-            a2 += g * (xv * xv);
-            a3 += g * (yv * yv);
-            a4 += g * (xv * yv);
-        }
-        hs[0][r][c] = a0;
-        hs[1][r][c] = a1;
-        hs[2][r][c] = a2;
-        hs[3][r][c] = a3;
-        hs[4][r][c] = a4;
+    {
+        const float *src[2] = {A.x, A.y};
+        const int64_t ps[2] = {A.xs[0], A.ys[0]}, rs[2] = {A.xs[1], A.ys[1]}, cs[2] = {A.xs[2], A.ys[2]};
+        stage_span<2>(sxy, src, ps, rs, cs, g0, A.P, A.W, A.H, x0, y0, tid);
     }
     __syncthreads();
 
     const int tx = tid % kTile, ty = tid / kTile;
-    float mu1 = 0.0f, mu2 = 0.0f, e11 = 0.0f, e22 = 0.0f, e12 = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kTaps; ++k) {
-        const float g = A.win[k];
-        mu1 += g * hs[0][ty + k][tx];
-        mu2 += g * hs[1][ty + k][tx];
-        e11 += g * hs[2][ty + k][tx];
-        e22 += g * hs[3][ty + k][tx];
-        e12 += g * hs[4][ty + k][tx];
-    }
     const int gx = bx * kTile + tx, gy = by * kTile + ty;
     const bool pix = gx < A.W && gy < A.H;
     const bool reg = in_ssim_region(A, gx, gy);
-    const float s11 = e11 - mu1 * mu1, s22 = e22 - mu2 * mu2, s12 = e12 - mu1 * mu2;
-    const float n1 = 2.0f * mu1 * mu2 + kC1, n2 = 2.0f * s12 + kC2;
-    const float d1 = mu1 * mu1 + mu2 * mu2 + kC1, d2 = s11 + s22 + kC2;
-    const float inv = 1.0f / (d1 * d2);
-    const float m = n1 * n2 * inv;
-
-    if (A.train && pix) {
-        float a = 0.0f, b = 0.0f, c = 0.0f;
-        if (reg) {
-            const float dmu1 = 2.0f * mu2 * n2 * inv - m * 2.0f * mu1 / d1;  // s11, s12 held
-            b = -m / d2;                                                     // dm/ds11
-            c = 2.0f * n1 * inv;                                             // dm/ds12
-            a = dmu1 - 2.0f * mu1 * b - mu2 * c;                             // dm/dmu1, E11 and E12 held
+    const int64_t plane3 = (int64_t)A.P * A.plane_px;
+    float l1 = 0.0f, ms = 0.0f;
+    for (int q = 0; q < np; ++q) {
+        float(*sx)[kSpan + 1] = sxy[q];
+        float(*sy)[kSpan + 1] = sxy[kGroup + q];
+        // horizontal pass: rows of the span, columns of the tile
+        for (int i = tid; i < kSpan * kTile; i += kThreads) {
+            const int r = i / kTile, c = i - r * kTile;
+            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f, a4 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kTaps; ++k) {
+                const float xv = sx[r][c + k], yv = sy[r][c + k], g = A.win[k];
+                a0 += g * xv;
+                a1 += g * yv;
+                a2 += g * (xv * xv);
+                a3 += g * (yv * yv);
+                a4 += g * (xv * yv);
+            }
+            hs[0][r][c] = a0;
+            hs[1][r][c] = a1;
+            hs[2][r][c] = a2;
+            hs[3][r][c] = a3;
+            hs[4][r][c] = a4;
         }
-        const int64_t o = (int64_t)p * A.plane_px + (int64_t)gy * A.W + gx;
-        const int64_t plane3 = (int64_t)A.P * A.plane_px;
-        A.maps[o] = a;
-        A.maps[plane3 + o] = b;
-        A.maps[2 * plane3 + o] = c;
+        __syncthreads();
+
+        float mu1 = 0.0f, mu2 = 0.0f, e11 = 0.0f, e22 = 0.0f, e12 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kTaps; ++k) {
+            const float g = A.win[k];
+            mu1 += g * hs[0][ty + k][tx];
+            mu2 += g * hs[1][ty + k][tx];
+            e11 += g * hs[2][ty + k][tx];
+            e22 += g * hs[3][ty + k][tx];
+            e12 += g * hs[4][ty + k][tx];
+        }
+        const float s11 = e11 - mu1 * mu1, s22 = e22 - mu2 * mu2, s12 = e12 - mu1 * mu2;
+        const float n1 = 2.0f * mu1 * mu2 + kC1, n2 = 2.0f * s12 + kC2;
+        const float d1 = mu1 * mu1 + mu2 * mu2 + kC1, d2 = s11 + s22 + kC2;
+        const float inv = 1.0f / (d1 * d2);
+        const float m = n1 * n2 * inv;
+        if (A.train && pix) {
+            float a = 0.0f, b = 0.0f, c = 0.0f;
+            if (reg) {
+                const float dmu1 = 2.0f * mu2 * n2 * inv - m * 2.0f * mu1 / d1;  // s11, s12 held
+                b = -m / d2;                                                     // dm/ds11
+                c = 2.0f * n1 * inv;                                             // dm/ds12
+                a = dmu1 - 2.0f * mu1 * b - mu2 * c;                             // dm/dmu1, E11 and E12 held
+            }
+            const int64_t o = (int64_t)(g0 + q) * A.plane_px + (int64_t)gy * A.W + gx;
+            A.maps[o] = a;
+            A.maps[plane3 + o] = b;
+            A.maps[2 * plane3 + o] = c;
+        }
+        if (pix) l1 += fabsf(sx[ty + kRadius][tx + kRadius] - sy[ty + kRadius][tx + kRadius]);
+        if (reg) ms += m;
+        if (q + 1 < np) __syncthreads();  // hs is rewritten by the next plane
     }
 
-    float l1 = pix ? fabsf(sx[ty + kRadius][tx + kRadius] - sy[ty + kRadius][tx + kRadius]) : 0.0f;
-    float ms = reg ? m : 0.0f;
     l1 = wave_sum(l1);
     ms = wave_sum(ms);
     if ((tid & 63) == 0) {
@@ -168,32 +211,41 @@ __global__ __launch_bounds__(kThreads) void k_loss_forward(LossArgs A) {
     }
     __syncthreads();
     if (tid == 0) {
-        const int64_t slot = (int64_t)p * A.tiles + tile;
+        const int64_t slot = (int64_t)blockIdx.y * A.tiles + tile;
         A.partials[2 * slot] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
         A.partials[2 * slot + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     }
 }
 
-__global__ __launch_bounds__(256) void k_loss_finish(LossArgs A) {
-    __shared__ double red[2][256];
-    const int64_t n = (int64_t)A.P * A.tiles;
+constexpr int kFinishThreads = 1024;
+
+__global__ __launch_bounds__(kFinishThreads) void k_loss_finish(LossArgs A) {
+    __shared__ double red[2][kFinishThreads / 64];
+    const int64_t n = (int64_t)A.groups * A.tiles;
+    const float2 *__restrict__ part = reinterpret_cast<const float2 *>(A.partials);
     double l1 = 0.0, ms = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) {
-        l1 += (double)A.partials[2 * i];
-        ms += (double)A.partials[2 * i + 1];
+    for (int64_t i = threadIdx.x; i < n; i += kFinishThreads) {
+        const float2 v = part[i];
+        l1 += (double)v.x;
+        ms += (double)v.y;
     }
-    red[0][threadIdx.x] = l1;
-    red[1][threadIdx.x] = ms;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        l1 += __shfl_xor(l1, o, 64);
+        ms += __shfl_xor(ms, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = l1;
+        red[1][threadIdx.x >> 6] = ms;
+    }
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            red[0][threadIdx.x] += red[0][threadIdx.x + s];
-            red[1][threadIdx.x] += red[1][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        const double l1m = red[0][0] / A.n_l1, ssim = red[1][0] / A.n_ssim;
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kFinishThreads / 64; ++w) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        const double l1m = a / A.n_l1, ssim = b / A.n_ssim;
         const double lam = (double)A.lambda;
         A.loss[0] = (float)((1.0 - lam) * l1m + lam * (1.0 - ssim));
         A.loss[1] = (float)l1m;
@@ -202,63 +254,63 @@ __global__ __launch_bounds__(256) void k_loss_finish(LossArgs A) {
 }
 
 __global__ __launch_bounds__(kThreads) void k_loss_backward(LossArgs A) {
-    __shared__ float sm[3][kSpan][kSpan + 1];
+    __shared__ float sm[3 * kGroup][kSpan][kSpan + 1];  // map-major: A of each plane, B of each, C of each
     __shared__ float hs[3][kSpan][kTile + 1];
 
     const int tile = tile_of(A);
     if (tile >= A.tiles) return;
-    const int p = blockIdx.y;
+    const int g0 = blockIdx.y * kGroup;
+    const int np = min(kGroup, A.P - g0);
     const int tid = threadIdx.x;
     const int bx = tile % A.tiles_x, by = tile / A.tiles_x;
     const int x0 = bx * kTile - kRadius, y0 = by * kTile - kRadius;
     const int64_t plane3 = (int64_t)A.P * A.plane_px;
-    const float *__restrict__ M = A.maps + (int64_t)p * A.plane_px;
-
-    for (int i = tid; i < kSpan * kSpan; i += kThreads) {
-        const int r = i / kSpan, c = i - r * kSpan;
-        const int gx = x0 + c, gy = y0 + r;
-        const bool in = gx >= 0 && gy >= 0 && gx < A.W && gy < A.H;
-        const int64_t o = (int64_t)gy * A.W + gx;
-        sm[0][r][c] = in ? M[o] : 0.0f;
-        sm[1][r][c] = in ? M[plane3 + o] : 0.0f;
-        sm[2][r][c] = in ? M[2 * plane3 + o] : 0.0f;
-    }
-    __syncthreads();
-
-    for (int i = tid; i < kSpan * kTile; i += kThreads) {
-        const int r = i / kTile, c = i - r * kTile;
-        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-#pragma unroll
-        for (int k = 0; k < kTaps; ++k) {
-            const float g = A.win[k];
-            a0 += g * sm[0][r][c + k];
-            a1 += g * sm[1][r][c + k];
-            a2 += g * sm[2][r][c + k];
-        }
-        hs[0][r][c] = a0;
-        hs[1][r][c] = a1;
-        hs[2][r][c] = a2;
+    {
+        const float *src[3] = {A.maps, A.maps + plane3, A.maps + 2 * plane3};
+        const int64_t ps[3] = {A.plane_px, A.plane_px, A.plane_px}, rs[3] = {A.W, A.W, A.W}, cs[3] = {1, 1, 1};
+        stage_span<3>(sm, src, ps, rs, cs, g0, A.P, A.W, A.H, x0, y0, tid);
     }
     __syncthreads();
 
     const int tx = tid % kTile, ty = tid / kTile;
     const int gx = bx * kTile + tx, gy = by * kTile + ty;
-    if (gx >= A.W || gy >= A.H) return;
-    float ga = 0.0f, gb = 0.0f, gc = 0.0f;
-#pragma unroll
-    for (int k = 0; k < kTaps; ++k) {
-        const float g = A.win[k];
-        ga += g * hs[0][ty + k][tx];
-        gb += g * hs[1][ty + k][tx];
-        gc += g * hs[2][ty + k][tx];
-    }
-    const float xv = A.x[(int64_t)p * A.xs[0] + gy * A.xs[1] + gx * A.xs[2]];
-    const float yv = A.y[(int64_t)p * A.ys[0] + gy * A.ys[1] + gx * A.ys[2]];
-    const float d = xv - yv;
-    const float sgn = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+    const bool pix = gx < A.W && gy < A.H;
     const float scale = A.scale_host * (A.scale_dev ? *A.scale_dev : 1.0f);
-    const float g = A.w_l1 * sgn + A.w_ssim * (ga + 2.0f * xv * gb + yv * gc);
-    A.grad[(int64_t)p * A.gs[0] + gy * A.gs[1] + gx * A.gs[2]] = scale * g;
+    for (int q = 0; q < np; ++q) {
+        for (int i = tid; i < kSpan * kTile; i += kThreads) {
+            const int r = i / kTile, c = i - r * kTile;
+            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kTaps; ++k) {
+                const float g = A.win[k];
+                a0 += g * sm[q][r][c + k];
+                a1 += g * sm[kGroup + q][r][c + k];
+                a2 += g * sm[2 * kGroup + q][r][c + k];
+            }
+            hs[0][r][c] = a0;
+            hs[1][r][c] = a1;
+            hs[2][r][c] = a2;
+        }
+        __syncthreads();
+        if (pix) {
+            float ga = 0.0f, gb = 0.0f, gc = 0.0f;
+#pragma unroll
+            for (int k = 0; k < kTaps; ++k) {
+                const float g = A.win[k];
+                ga += g * hs[0][ty + k][tx];
+                gb += g * hs[1][ty + k][tx];
+                gc += g * hs[2][ty + k][tx];
+            }
+            const int64_t p = g0 + q;
+            const float xv = A.x[p * A.xs[0] + gy * A.xs[1] + gx * A.xs[2]];
+            const float yv = A.y[p * A.ys[0] + gy * A.ys[1] + gx * A.ys[2]];
+            const float d = xv - yv;
+            const float sgn = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+            const float g = A.w_l1 * sgn + A.w_ssim * (ga + 2.0f * xv * gb + yv * gc);
+            A.grad[p * A.gs[0] + gy * A.gs[1] + gx * A.gs[2]] = scale * g;
+        }
+        if (q + 1 < np) __syncthreads();
+    }
 }
 
 int64_t maps_bytes(int64_t P, int64_t H, int64_t W) { return (3 * P * H * W * 4 + 255) / 256 * 256; }
@@ -270,7 +322,7 @@ int setup(const gray_loss_args *a, LossArgs &A) {
         gray_set_error(GRAY_ERR_INVALID, "loss: null args, render, target or workspace");
         return GRAY_ERR_INVALID;
     }
-    if (a->planes <= 0 || a->height <= 0 || a->width <= 0 || a->planes > 65535) {
+    if (a->planes <= 0 || a->height <= 0 || a->width <= 0 || a->planes > 3 * 65535) {
         gray_set_error(GRAY_ERR_INVALID, "loss: bad shape planes=%d height=%d width=%d", a->planes, a->height,
                        a->width);
         return GRAY_ERR_INVALID;
@@ -302,6 +354,7 @@ int setup(const gray_loss_args *a, LossArgs &A) {
     A.tiles_x = (a->width + kTile - 1) / kTile;
     A.tiles = (int32_t)tiles_of(a->height, a->width);
     A.xcd_chunk = (A.tiles + 7) / 8;
+    A.groups = (a->planes + kGroup - 1) / kGroup;
     // normalised Gaussian window, sigma 1.5, formed in double
     double w[kTaps], sum = 0.0;
     for (int k = 0; k < kTaps; ++k) {
@@ -328,7 +381,7 @@ extern "C" int gray_loss_workspace_bytes(int32_t planes, int32_t height, int32_t
         gray_set_error(GRAY_ERR_INVALID, "loss workspace: bad shape or null output");
         return GRAY_ERR_INVALID;
     }
-    *bytes = maps_bytes(planes, height, width) + 2 * 4 * (int64_t)planes * tiles_of(height, width);
+    *bytes = maps_bytes(planes, height, width) + 2 * 4 * (int64_t)((planes + kGroup - 1) / kGroup) * tiles_of(height, width);
     return GRAY_OK;
 }
 
@@ -340,9 +393,9 @@ extern "C" int gray_loss_forward(const gray_loss_args *args, void *stream) {
         return GRAY_ERR_INVALID;
     }
     const hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_loss_forward, dim3(8 * A.xcd_chunk, A.P), dim3(kThreads), 0, s, A);
+    hipLaunchKernelGGL(k_loss_forward, dim3(8 * A.xcd_chunk, A.groups), dim3(kThreads), 0, s, A);
     GRAY_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(256), 0, s, A);
+    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(kFinishThreads), 0, s, A);
     GRAY_HIP(hipGetLastError());
     return GRAY_OK;
 }
@@ -354,7 +407,7 @@ extern "C" int gray_loss_backward(const gray_loss_args *args, void *stream) {
         gray_set_error(GRAY_ERR_INVALID, "loss backward: null gradient output");
         return GRAY_ERR_INVALID;
     }
-    hipLaunchKernelGGL(k_loss_backward, dim3(8 * A.xcd_chunk, A.P), dim3(kThreads), 0, (hipStream_t)stream, A);
+    hipLaunchKernelGGL(k_loss_backward, dim3(8 * A.xcd_chunk, A.groups), dim3(kThreads), 0, (hipStream_t)stream, A);
     GRAY_HIP(hipGetLastError());
     return GRAY_OK;
 }

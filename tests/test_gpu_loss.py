@@ -30,6 +30,7 @@ def pair(shape, seed):
     ((3, 64, 48), 1.0, "valid"),     # SSIM only, border excluded
     ((2, 3, 40, 33), 0.2, "valid"),  # batch
     ((1, 1, 1), 0.2, "same"),        # one pixel
+    ((5, 30, 21), 0.2, "same"),      # a full and a partial group of planes
 ])
 def test_loss_and_gradient_match_oracle(shape, lam, padding):
     x, y = pair(shape, seed=len(shape) * 1000 + shape[-1])

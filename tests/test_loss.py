@@ -90,7 +90,7 @@ def test_host_checks_without_device():
     assert L.LIB.gray_loss_workspace_bytes(0, 4, 4, C.byref(n)) == L.GRAY_ERR_INVALID
     assert L.LIB.gray_loss_workspace_bytes(3, 545, 980, C.byref(n)) == L.GRAY_OK
     tiles = 62 * 35
-    assert n.value == (3 * 3 * 545 * 980 * 4 + 255) // 256 * 256 + 2 * 4 * 3 * tiles
+    assert n.value == (3 * 3 * 545 * 980 * 4 + 255) // 256 * 256 + 2 * 4 * tiles  # one group of 3 planes
     assert L.LIB.gray_loss_forward(None, None) == L.GRAY_ERR_INVALID
     a = L.LossArgs()
     a.planes, a.height, a.width, a.valid_padding = 3, 8, 8, 1
