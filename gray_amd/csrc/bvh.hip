@@ -9,8 +9,9 @@
 //         (8-bit digits, LDS histograms, wave-ballot stable ranking) ->
 //         Karras 2012 topology -> refit.
 // Refit:  phase 1: one workgroup per 1024-leaf chunk refits every node whose
-//         leaf range lies inside the chunk, bottom-up with LDS counters, and
-//         writes chunk prefix/suffix box scans; phase 2: sparse table over the
+//         leaf range lies inside the chunk, bottom-up with LDS counters over
+//         LDS-staged node metadata, and writes the chunk prefix/suffix boxes
+//         that spanning nodes read; phase 2: sparse table over the
 //         chunk boxes (one workgroup); phase 3: every node spanning chunks
 //         takes suffix(a) | table(chunks) | prefix(b) independently.  No kernel
 //         depends on another workgroup's plain stores within the same launch.
@@ -63,7 +64,8 @@ __global__ void k_snapshot(int n, const float *__restrict__ mean, const float *_
     const float sz = (expf(scale[3 * g + 2]) * clip) * gsf;
     const bool visible = clip > 0.0f && (sx > 0.0f || sy > 0.0f || sz > 0.0f);
 
-    float r = rotation[4 * g + 0], x = rotation[4 * g + 1], y = rotation[4 * g + 2], z = rotation[4 * g + 3];
+    const float4 q = reinterpret_cast<const float4 *>(rotation)[g];
+    float r = q.x, x = q.y, y = q.z, z = q.w;
     const float norm = sqrtf(r * r + x * x + y * y + z * z);
     r /= norm;
     x /= norm;
@@ -100,9 +102,10 @@ __global__ void k_snapshot(int n, const float *__restrict__ mean, const float *_
     rec.clip = clip;
     rec.pad0 = rec.pad1 = 0.0f;
     geo[g] = rec;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) l2w[12 * g + i] = A[i];
-    l2w[12 * g + 9] = l2w[12 * g + 10] = l2w[12 * g + 11] = 0.0f;
+    float4 *L = reinterpret_cast<float4 *>(l2w + 12 * (size_t)g);
+    L[0] = make_float4(A[0], A[1], A[2], A[3]);
+    L[1] = make_float4(A[4], A[5], A[6], A[7]);
+    L[2] = make_float4(A[8], 0.0f, 0.0f, 0.0f);
 
     float box[6];
     if (visible) {
@@ -117,8 +120,10 @@ __global__ void k_snapshot(int n, const float *__restrict__ mean, const float *_
     } else {
         box_empty(box);
     }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) aabb[6 * g + i] = box[i];
+    float2 *ab = reinterpret_cast<float2 *>(aabb + 6 * (size_t)g);
+    ab[0] = make_float2(box[0], box[1]);
+    ab[1] = make_float2(box[2], box[3]);
+    ab[2] = make_float2(box[4], box[5]);
 }
 
 // ---------------------------------------------------------------------------
@@ -344,7 +349,8 @@ __global__ void k_karras(int n, const uint32_t *__restrict__ keys, const uint32_
                          BvhNode *__restrict__ nodes, uint32_t *__restrict__ parent_int,
                          uint32_t *__restrict__ parent_leaf, uint32_t *__restrict__ range_lo,
                          uint32_t *__restrict__ range_hi, uint32_t *__restrict__ top_nodes,
-                         uint32_t *__restrict__ top_count) {
+                         uint32_t *__restrict__ top_count, uint8_t *__restrict__ need_ends,
+                         uint4 *__restrict__ node_info) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int d = (kdelta(keys, n, i, i + 1) - kdelta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -370,6 +376,7 @@ __global__ void k_karras(int n, const uint32_t *__restrict__ keys, const uint32_
     } else {
         left = (uint32_t)gamma;
         parent_int[gamma] = (uint32_t)i;
+        node_info[gamma].w = (uint32_t)i;
     }
     if (hi == gamma + 1) {
         right = GRAY_LEAF | sorted_ids[gamma + 1];
@@ -377,7 +384,13 @@ __global__ void k_karras(int n, const uint32_t *__restrict__ keys, const uint32_
     } else {
         right = (uint32_t)(gamma + 1);
         parent_int[gamma + 1] = (uint32_t)i | 0x80000000u;
+        node_info[gamma + 1].w = (uint32_t)i | 0x80000000u;
     }
+    // .w is the parent, written by the parent's thread
+    node_info[i].x = (uint32_t)lo;
+    node_info[i].y = (uint32_t)hi;
+    node_info[i].z = (uint32_t)gamma;
+    if (i == 0) node_info[0].w = GRAY_EMPTY;
     nodes[i].left = left;
     nodes[i].right = right;
     nodes[i].pad0 = (uint32_t)lo;
@@ -388,6 +401,8 @@ __global__ void k_karras(int n, const uint32_t *__restrict__ keys, const uint32_
     if (lo / GRAY_CHUNK != hi / GRAY_CHUNK) {
         const uint32_t slot = atomicAdd(top_count, 1u);
         top_nodes[slot] = (uint32_t)i;
+        need_ends[2 * lo] = 1;      // suffix of lo within its chunk
+        need_ends[2 * hi + 1] = 1;  // prefix of hi within its chunk
     }
 }
 
@@ -415,136 +430,133 @@ __global__ void k_single_leaf(BvhNode *__restrict__ nodes, uint32_t *__restrict_
 // ---------------------------------------------------------------------------
 // Refit, phase 1: per chunk.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_refit_chunks(int n, const float *__restrict__ aabb,
-                                                      const uint32_t *__restrict__ sorted_ids,
-                                                      const uint32_t *__restrict__ parent_int,
-                                                      const uint32_t *__restrict__ parent_leaf,
-                                                      const uint32_t *__restrict__ range_lo,
-                                                      const uint32_t *__restrict__ range_hi,
-                                                      BvhNode *__restrict__ nodes, float *__restrict__ prefix,
-                                                      float *__restrict__ suffix, float *__restrict__ chunk_box) {
+__device__ __forceinline__ void store_box(float *dst, const float *b) {
+    // node boxes sit at 8-byte aligned offsets (0 and 24 of a 64-byte node)
+    float2 *d = reinterpret_cast<float2 *>(dst);
+    d[0] = make_float2(b[0], b[1]);
+    d[1] = make_float2(b[2], b[3]);
+    d[2] = make_float2(b[4], b[5]);
+}
+
+__device__ __forceinline__ void shfl_box(float *out, const float *b, int delta, bool up) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) out[i] = up ? __shfl_up(b[i], delta, 64) : __shfl_down(b[i], delta, 64);
+}
+
+// One workgroup per chunk of GRAY_CHUNK consecutive leaves, one leaf per
+// thread.  The chunk's leaf boxes, the metadata of its internal nodes (children,
+// leaf range, parent; Karras node i lies in the chunk of leaf i) and the
+// arrival counters are staged in LDS, so a walk up from a leaf touches global
+// memory only to write the two child boxes of each node it completes (the
+// second arrival) and, at the first node spanning chunks, that node's slot.
+// The chunk's prefix and suffix unions are formed with wave shuffles and are
+// written only at the leaf positions a spanning node reads (need_ends).
+__global__ __launch_bounds__(GRAY_CHUNK) void k_refit_chunks(int n, const float *__restrict__ aabb,
+                                                             const uint32_t *__restrict__ sorted_ids,
+                                                             const uint4 *__restrict__ node_info,
+                                                             const uint32_t *__restrict__ parent_leaf,
+                                                             const uint8_t *__restrict__ need_ends,
+                                                             BvhNode *__restrict__ nodes, float *__restrict__ prefix,
+                                                             float *__restrict__ suffix,
+                                                             float *__restrict__ chunk_box) {
     constexpr int B = GRAY_CHUNK;
-    constexpr int PER = B / 256;
+    constexpr int kWaves = B / 64;
     __shared__ float leafb[B][6];
     __shared__ float intb[B][6];
+    __shared__ uint4 meta[B];  // range lo, range hi, split, parent of internal node base + k
     __shared__ uint32_t cnt[B];
-    __shared__ float scan[256][6];
-    const int tid = threadIdx.x;
+    __shared__ float wtot[2][kWaves][6];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int base = blockIdx.x * B;
-    const int cend = min(n, base + B);  // exclusive
-    for (int k = tid; k < B; k += 256) {
-        cnt[k] = 0;
-        const int leaf = base + k;
-        if (leaf < cend) {
-            const uint32_t g = sorted_ids[leaf];
-            for (int i = 0; i < 6; ++i) leafb[k][i] = aabb[6 * g + i];
-        } else {
-            box_empty(leafb[k]);
-        }
+    const int leaf = base + t;
+    const bool live = leaf < n;
+    float box[6];
+    uint32_t pref = GRAY_EMPTY;
+    uchar2 need = make_uchar2(0, 0);
+    if (live) {
+        const float2 *a2 = reinterpret_cast<const float2 *>(aabb + 6 * (size_t)sorted_ids[leaf]);
+        const float2 v0 = a2[0], v1 = a2[1], v2 = a2[2];
+        box[0] = v0.x;
+        box[1] = v0.y;
+        box[2] = v1.x;
+        box[3] = v1.y;
+        box[4] = v2.x;
+        box[5] = v2.y;
+        pref = parent_leaf[leaf];
+        need = reinterpret_cast<const uchar2 *>(need_ends)[leaf];
+    } else {
+        box_empty(box);
     }
-    __syncthreads();
+    if (leaf < n - 1) meta[t] = node_info[leaf];
+    cnt[t] = 0u;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) leafb[t][i] = box[i];
 
-    // prefix / suffix unions inside the chunk (thread owns PER consecutive leaves)
-    float run[6];
-    box_empty(run);
-    for (int j = 0; j < PER; ++j) box_union(run, leafb[tid * PER + j]);
-    for (int i = 0; i < 6; ++i) scan[tid][i] = run[i];
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        float tmp[6];
-        const bool has = tid >= off;
-        if (has)
-            for (int i = 0; i < 6; ++i) tmp[i] = scan[tid - off][i];
-        __syncthreads();
-        if (has) box_union(scan[tid], tmp);
-        __syncthreads();
+    // prefix / suffix unions within the wave, then across the waves
+    float pre[6], suf[6], o[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) pre[i] = suf[i] = box[i];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        shfl_box(o, pre, d, true);
+        if (lane >= d) box_union(pre, o);
+        shfl_box(o, suf, d, false);
+        if (lane + d < 64) box_union(suf, o);
     }
-    {
-        float acc[6];
-        if (tid > 0)
-            for (int i = 0; i < 6; ++i) acc[i] = scan[tid - 1][i];
-        else
-            box_empty(acc);
-        for (int j = 0; j < PER; ++j) {
-            const int k = tid * PER + j;
-            box_union(acc, leafb[k]);
-            if (base + k < cend)
-                for (int i = 0; i < 6; ++i) prefix[6 * (size_t)(base + k) + i] = acc[i];
-        }
-        if (tid == 255)
-            for (int i = 0; i < 6; ++i) chunk_box[6 * blockIdx.x + i] = scan[255][i];
-    }
+    if (lane == 63)
+        for (int i = 0; i < 6; ++i) wtot[0][wave][i] = pre[i];
+    if (lane == 0)
+        for (int i = 0; i < 6; ++i) wtot[1][wave][i] = suf[i];
     __syncthreads();
-    // suffix: reverse scan
-    box_empty(run);
-    for (int j = 0; j < PER; ++j) box_union(run, leafb[tid * PER + j]);
-    for (int i = 0; i < 6; ++i) scan[tid][i] = run[i];
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        float tmp[6];
-        const bool has = tid + off < 256;
-        if (has)
-            for (int i = 0; i < 6; ++i) tmp[i] = scan[tid + off][i];
-        __syncthreads();
-        if (has) box_union(scan[tid], tmp);
-        __syncthreads();
-    }
-    {
-        float acc[6];
-        if (tid < 255)
-            for (int i = 0; i < 6; ++i) acc[i] = scan[tid + 1][i];
-        else
-            box_empty(acc);
-        for (int j = PER - 1; j >= 0; --j) {
-            const int k = tid * PER + j;
-            box_union(acc, leafb[k]);
-            if (base + k < cend)
-                for (int i = 0; i < 6; ++i) suffix[6 * (size_t)(base + k) + i] = acc[i];
-        }
-    }
-    __syncthreads();
+    for (int w = 0; w < wave; ++w) box_union(pre, wtot[0][w]);
+    for (int w = wave + 1; w < kWaves; ++w) box_union(suf, wtot[1][w]);
+    if (need.x) store_box(suffix + 6 * (size_t)leaf, suf);
+    if (need.y) store_box(prefix + 6 * (size_t)leaf, pre);
+    if (t == 0) store_box(chunk_box + 6 * (size_t)blockIdx.x, suf);
 
     // bottom-up over the nodes whose range lies inside this chunk
-    for (int k = tid; k < B; k += 256) {
-        const int leaf = base + k;
-        if (leaf >= cend) continue;
-        float cur[6];
-        for (int i = 0; i < 6; ++i) cur[i] = leafb[k][i];
-        uint32_t pref = parent_leaf[leaf];
-        bool child_is_leaf = true;
-        int child = leaf;
-        while (pref != GRAY_EMPTY) {
-            const uint32_t p = pref & 0x7fffffffu;
-            const bool right = (pref & 0x80000000u) != 0u;
-            float *slot = right ? nodes[p].rb : nodes[p].lb;
-            for (int i = 0; i < 6; ++i) slot[i] = cur[i];
-            const uint32_t lo = range_lo[p], hi = range_hi[p];
-            const bool inside = (int)lo >= base && (int)hi < base + B;
-            if (!inside) break;  // phase 3 finishes nodes spanning chunks
-            if (!child_is_leaf) {
-                for (int i = 0; i < 6; ++i) intb[child - base][i] = cur[i];
-            }
-            __threadfence_block();
-            const uint32_t old = atomicAdd(&cnt[p - base], 1u);
-            if (old == 0u) break;
-            __threadfence_block();
-            // second arrival: sibling box from LDS
-            const uint32_t sib = right ? nodes[p].left : nodes[p].right;
-            const float *sb;
-            if (sib == GRAY_EMPTY) {
-                sb = nullptr;
-            } else if (sib & GRAY_LEAF) {
-                // sibling leaf position: the neighbour of this child's range end
-                const int spos = right ? (int)lo : (int)hi;
-                sb = leafb[spos - base];
-            } else {
-                sb = intb[sib - base];
-            }
-            if (sb) box_union(cur, sb);
-            child_is_leaf = false;
-            child = (int)p;
-            pref = parent_int[p];
+    if (!live) return;
+    if (n == 1) {  // the root's only child
+        float e[6];
+        box_empty(e);
+        store_box(nodes[0].lb, box);
+        store_box(nodes[0].rb, e);
+        return;
+    }
+    float cur[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) cur[i] = box[i];
+    bool child_is_leaf = true;
+    int child = leaf;
+    while (pref != GRAY_EMPTY) {
+        const int p = (int)(pref & 0x7fffffffu);
+        const bool right = (pref & 0x80000000u) != 0u;
+        const bool in_chunk = p >= base && p < base + B;
+        const uint4 m = in_chunk ? meta[p - base] : make_uint4(0u, 0u, 0u, 0u);
+        if (!in_chunk || (int)m.x < base || (int)m.y >= base + B) {
+            // p spans chunks: its slot for this child now; the rest of p is phase 3's
+            store_box(right ? nodes[p].rb : nodes[p].lb, cur);
+            break;
         }
+        if (!child_is_leaf)
+            for (int i = 0; i < 6; ++i) intb[child - base][i] = cur[i];
+        __threadfence_block();
+        if (atomicAdd(&cnt[p - base], 1u) == 0u) break;
+        __threadfence_block();
+        // second arrival: the sibling's box is in LDS.  Children of a Karras
+        // node split its range at m.z: left = leaf m.z if lo == m.z, else node
+        // m.z; right = leaf m.z + 1 if hi == m.z + 1, else node m.z + 1.
+        const int sidx = right ? (int)m.z : (int)m.z + 1;
+        const bool sleaf = right ? m.x == m.z : m.y == m.z + 1;
+        const float *sb = sleaf ? leafb[sidx - base] : intb[sidx - base];
+        float sbox[6];
+        for (int i = 0; i < 6; ++i) sbox[i] = sb[i];
+        store_box(nodes[p].lb, right ? sbox : cur);
+        store_box(nodes[p].rb, right ? cur : sbox);
+        box_union(cur, sbox);
+        child_is_leaf = false;
+        child = p;
+        pref = m.w;
     }
 }
 
@@ -639,6 +651,8 @@ int gray_ctx_reserve(gray_ctx *ctx, int64_t n) {
     G(prefix, cap * 6);
     G(suffix, cap * 6);
     G(top_nodes, cap);
+    G(need_ends, cap * 2);
+    G(node_info, cap);
     const int64_t tiles = gray_div_up(cap, kSortTile);
     const int64_t hm = 256 * tiles;
     G(hist, hm);
@@ -657,9 +671,9 @@ int gray_refit(gray_ctx *ctx, hipStream_t s) {
     const int n = (int)ctx->n;
     if (n <= 0) return GRAY_OK;
     const int chunks = chunks_of(n);
-    hipLaunchKernelGGL(k_refit_chunks, dim3(chunks), dim3(256), 0, s, n, ctx->aabb, ctx->sorted_ids,
-                       ctx->parent_int, ctx->parent_leaf, ctx->range_lo, ctx->range_hi, ctx->nodes, ctx->prefix,
-                       ctx->suffix, ctx->sparse);
+    hipLaunchKernelGGL(k_refit_chunks, dim3(chunks), dim3(GRAY_CHUNK), 0, s, n, ctx->aabb, ctx->sorted_ids,
+                       ctx->node_info, ctx->parent_leaf, ctx->need_ends, ctx->nodes, ctx->prefix, ctx->suffix,
+                       ctx->sparse);
     GRAY_HIP(hipGetLastError());
     if (ctx->num_top > 0) {
         const int levels = levels_of(chunks);
@@ -675,6 +689,7 @@ int gray_refit(gray_ctx *ctx, hipStream_t s) {
 
 static int build_topology(gray_ctx *ctx, const float *mean, hipStream_t s) {
     const int n = (int)ctx->n;
+    GRAY_HIP(hipMemsetAsync(ctx->need_ends, 0, 2 * (size_t)n, s));
     if (n == 1) {
         hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(1), 0, s, ctx->nodes, ctx->parent_int, ctx->parent_leaf,
                            ctx->range_lo, ctx->range_hi, ctx->sorted_ids, ctx->keys);
@@ -710,7 +725,7 @@ static int build_topology(gray_ctx *ctx, const float *mean, hipStream_t s) {
     GRAY_HIP(hipMemsetAsync(ctx->top_count, 0, sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_karras, dim3(gray_div_up(n - 1, 256)), dim3(256), 0, s, n, ctx->keys, ctx->sorted_ids,
                        ctx->nodes, ctx->parent_int, ctx->parent_leaf, ctx->range_lo, ctx->range_hi, ctx->top_nodes,
-                       ctx->top_count);
+                       ctx->top_count, ctx->need_ends, ctx->node_info);
     GRAY_HIP(hipGetLastError());
     uint32_t top = 0;
     GRAY_HIP(hipMemcpyAsync(&top, ctx->top_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -732,6 +747,10 @@ extern "C" int gray_bvh_update(gray_ctx *ctx, const gray_params *params, const g
     }
     if (n > 0 && (!params->mean || !params->rotation || !params->scale || !params->opacity)) {
         gray_set_error(GRAY_ERR_INVALID, "gray_bvh_update: null parameter pointer");
+        return GRAY_ERR_INVALID;
+    }
+    if (((uintptr_t)params->rotation & 15u) != 0u) {
+        gray_set_error(GRAY_ERR_INVALID, "gray_bvh_update: rotation must be 16-byte aligned (one float4 per Gaussian)");
         return GRAY_ERR_INVALID;
     }
     hipStream_t s = (hipStream_t)stream;

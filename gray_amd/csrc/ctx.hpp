@@ -20,6 +20,7 @@ struct gray_ctx {
     uint32_t *parent_leaf = nullptr;// [cap]
     uint32_t *range_lo = nullptr;   // [cap]
     uint32_t *range_hi = nullptr;   // [cap]
+    uint4 *node_info = nullptr;     // [cap]: leaf range lo, hi, split, parent (bit 31: right child)
     uint32_t *sorted_ids = nullptr; // [cap] leaf k -> Gaussian id
     uint32_t *keys = nullptr;       // [cap] sorted Morton codes
     // radix sort scratch
@@ -30,7 +31,8 @@ struct gray_ctx {
     float *bounds_partial = nullptr;// [blocks*6]
     float *bounds = nullptr;        // [6]
     // refit scratch
-    float *prefix = nullptr, *suffix = nullptr; // [cap,6]
+    float *prefix = nullptr, *suffix = nullptr; // [cap,6], written only where need_ends is set
+    uint8_t *need_ends = nullptr;   // [cap][2]: suffix / prefix box read by a node spanning chunks
     float *sparse = nullptr;        // [levels, chunks, 6]
     int64_t sparse_cap = 0;
     uint32_t *top_nodes = nullptr;  // [cap]

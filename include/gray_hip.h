@@ -62,7 +62,7 @@ typedef struct {
     int32_t sh_is_fp16;       /* inference_only layout (gaussians.h:132) */
     const int32_t *current_sh_degree; /* [1] device */
     const float *mean;        /* [N,3] */
-    const float *rotation;    /* [N,4] */
+    const float *rotation;    /* [N,4], 16-byte aligned */
     const float *scale;       /* [N,3] log-scale */
     const float *opacity;     /* [N,1] logit */
     const void *sh_dc;        /* [N,1,3] fp32 (or fp16 when sh_is_fp16) */
