@@ -97,7 +97,8 @@ struct TraceArgs {
     // continuation tasks of the collect pass, {lane g, node or leaf reference}
     uint2 *task_in, *task_out;
     int32_t *task_count_in, *task_count_out, *task_head;
-    int32_t task_cap, pad4;
+    int32_t task_cap;
+    int32_t fp16;  // fp16_hits: distances and compositing alphas rounded to fp16 as the reference stores them
 };
 
 // ---------------------------------------------------------------------------

@@ -32,6 +32,8 @@
 
 #include <algorithm>
 
+#include <hip/hip_fp16.h>
+
 #include "ctx.hpp"
 
 namespace {
@@ -357,6 +359,15 @@ constexpr int kTaskWaves = 4096;           // persistent waves of a task round
 constexpr int32_t kRayDead = -(1 << 30);   // list_n of a ray whose list or tasks overflowed
 static_assert((kLeafQueue & (kLeafQueue - 1)) == 0 && kLeafQueue >= 2 * kCollectHeads, "leaf queue size");
 
+// fp16_hits: the reference keeps each hit's distance and alpha as fp16
+// (per_pixel_linked_list.h:28-29, __float2half_rn) and orders, cuts and
+// composites with those values (forward_pass.cu:53-110); its full
+// transmittance multiplies the fp32 alpha of the hit program.  Here the
+// distance is rounded where a hit is produced and the alpha where it is
+// composited or summed, so the fp32 alpha stays available for the full
+// transmittance.
+__device__ __forceinline__ float fp16_round(float x, bool on) { return on ? __half2float(__float2half_rn(x)) : x; }
+
 __device__ __forceinline__ void put_hit(const TraceArgs &A, size_t g, int slot, float dist, uint32_t gid, float a) {
     const size_t e = (size_t)slot * (size_t)A.list_stride + g;
     A.list_rec[e] = make_float4(dist, __uint_as_float(gid), a, 0.0f);
@@ -451,6 +462,7 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
                 continue;
             float gv, qq;
             const float a = gray_alpha(hx, hy, hz, q.clip, q.op, p, gv, qq);
+            dist = fp16_round(dist, A.fp16 != 0);
             if (kTask) {
                 const int slot = atomicAdd(A.list_n + g, 1);
                 if (slot >= 0 && slot < cap) put_hit(A, g, slot, dist, pl[l], a);
@@ -629,10 +641,11 @@ __global__ void k_zero_i32(int32_t *p, int n) {
 // with the truncation estimate and background, depth, and the backward state.
 // skc/ska/pnc are the skip-set sums (sum a*c, sum a, prod (1-a)).
 __device__ __forceinline__ void store_pixel(const TraceArgs &A, const DevView &V, int pix, const LaneRay &r,
-                                            bool depth_on, float T, float pnc, float skc0, float skc1, float skc2,
-                                            float ska, float C0, float C1, float C2, float D, uint32_t ncomp,
-                                            uint32_t nhits) {
-    const float F = T * pnc;
+                                            bool depth_on, float T, float Tfull, float pnc, float skc0, float skc1,
+                                            float skc2, float ska, float C0, float C1, float C2, float D,
+                                            uint32_t ncomp, uint32_t nhits) {
+    // full transmittance: composited hits (Tfull, fp32 alphas) times the skip set
+    const float F = Tfull * pnc;
     const float rem = T - F;
     float e0 = 0.0f, e1 = 0.0f, e2 = 0.0f;
     if (ska > 0.0f) {
@@ -719,7 +732,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
     const float *__restrict__ rgb = rgb0 + (V.rgb - A.views[0].rgb);
 
     KBuf kb{s_kd, s_ki, s_ka, lane};
-    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
+    const bool h16 = A.fp16 != 0;
+    float T = 1.0f, Tf = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
     float skc0 = 0.0f, skc1 = 0.0f, skc2 = 0.0f, ska = 0.0f, pnc = 1.0f;
     float tmin = znear;
     uint32_t ncomp = 0, nhits = 0;
@@ -749,16 +763,18 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
             if (beyond) res = kb_insert(kb, count, lastd, lasti, dist, gid, a, ev_id, ev_a);
             if (!first) return;
             if (res == 0) {
-                skc0 += a * c0;
-                skc1 += a * c1;
-                skc2 += a * c2;
-                ska += a;
+                const float ah = fp16_round(a, h16);
+                skc0 += ah * c0;
+                skc1 += ah * c1;
+                skc2 += ah * c2;
+                ska += ah;
                 pnc = pnc * (1.0f - a);
             } else if (res == 2) {
-                skc0 += ev_a * rgb[3 * ev_id + 0];
-                skc1 += ev_a * rgb[3 * ev_id + 1];
-                skc2 += ev_a * rgb[3 * ev_id + 2];
-                ska += ev_a;
+                const float ah = fp16_round(ev_a, h16);
+                skc0 += ah * rgb[3 * ev_id + 0];
+                skc1 += ah * rgb[3 * ev_id + 1];
+                skc2 += ah * rgb[3 * ev_id + 2];
+                ska += ah;
                 pnc = pnc * (1.0f - ev_a);
             }
         };
@@ -771,7 +787,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
                     return;
                 float gv, q;
                 const float a = gray_alpha(hx, hy, hz, g.clip, g.op, p, gv, q);
-                found(gid, dist, a, c0, c1, c2);
+                found(gid, fp16_round(dist, h16), a, c0, c1, c2);
             };
             traverse(A, nodes, geo, rgb, first, r, part, first ? 0.0f : cur_d, s_stack, lane, leaf,
                      A.tile_stats ? visits : nullptr);
@@ -783,14 +799,15 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
             for (int k = 0; k < count; ++k) {
                 const float d = kb.dist(k);
                 const uint32_t gid = kb.gid(k);
-                const float a = kb.alpha(k);
+                const float a32 = kb.alpha(k);
+                const float a = fp16_round(a32, h16);
                 if (terminated) {
                     if (first) {
                         skc0 += a * rgb[3 * gid + 0];
                         skc1 += a * rgb[3 * gid + 1];
                         skc2 += a * rgb[3 * gid + 2];
                         ska += a;
-                        pnc = pnc * (1.0f - a);
+                        pnc = pnc * (1.0f - a32);
                     }
                     continue;
                 }
@@ -802,7 +819,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
                         skc1 -= a * c1;
                         skc2 -= a * c2;
                         ska -= a;
-                        pnc = pnc / (1.0f - a);
+                        pnc = pnc / (1.0f - a32);
                     }
                     const float w = T * a;
                     C0 += c0 * w;
@@ -814,6 +831,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
                         V.hit_T[(size_t)ncomp * R + pix] = T;
                     }
                     T = T * (1.0f - a);
+                    Tf = Tf * (1.0f - a32);
                     ++ncomp;
                     if (T < tthr) {
                         tmin = zfar;
@@ -824,7 +842,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
                     skc1 += a * c1;
                     skc2 += a * c2;
                     ska += a;
-                    pnc = pnc * (1.0f - a);
+                    pnc = pnc * (1.0f - a32);
                 }
             }
             alive = false;
@@ -868,7 +886,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
         }
     }
     if (!active) return;
-    store_pixel(A, V, pix, r, depth_on, T, pnc, skc0, skc1, skc2, ska, C0, C1, C2, D, ncomp, nhits);
+    store_pixel(A, V, pix, r, depth_on, T, Tf, pnc, skc0, skc1, skc2, ska, C0, C1, C2, D, ncomp, nhits);
 }
 
 // ---------------------------------------------------------------------------
@@ -946,7 +964,8 @@ __global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) 
     const float *__restrict__ rgb = rgb0 + (V.rgb - A.views[0].rgb);
 
     RegKB kb;
-    float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
+    const bool h16 = A.fp16 != 0;
+    float T = 1.0f, Tf = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
     float tmin = znear;
     uint32_t ncomp = 0;
     const bool traced = active && tmin < zfar;
@@ -1001,7 +1020,8 @@ __global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) 
             if (!(dk < zfar)) continue;
             const uint32_t gid = kb.id[k];
             const size_t e = (size_t)kb.slot[k] * S + lg;
-            const float a = L[e].z;
+            const float a32 = L[e].z;
+            const float a = fp16_round(a32, h16);
             const float c0 = rgb[3 * gid + 0], c1 = rgb[3 * gid + 1], c2 = rgb[3 * gid + 2];
             const float w = T * a;
             C0 += c0 * w;
@@ -1014,6 +1034,7 @@ __global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) 
             }
             reinterpret_cast<uint32_t *>(L + e)[1] = gid | kListMark;
             T = T * (1.0f - a);
+            Tf = Tf * (1.0f - a32);
             ++ncomp;
             if (T < tthr) {
                 tmin = zfar;
@@ -1061,7 +1082,7 @@ __global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
         for (int j = 0; j < kListBatch; ++j) {
             const uint32_t id = __float_as_uint(sx[j].x);
-            w[j] = (id & kListMark) ? 0.0f : sx[j].y;
+            w[j] = (id & kListMark) ? 0.0f : sx[j].y;  // fp32 alpha; its fp16 rounding enters the sums below
             const uint32_t gid = id & ~kListMark;
             c[j][0] = rgb[3 * gid + 0];
             c[j][1] = rgb[3 * gid + 1];
@@ -1071,16 +1092,17 @@ __global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) 
 #pragma unroll
         for (int j = 0; j < kListBatch; ++j) {
             if (w[j] == 0.0f) continue;
-            skc0 += w[j] * c[j][0];
-            skc1 += w[j] * c[j][1];
-            skc2 += w[j] * c[j][2];
-            ska += w[j];
+            const float wh = fp16_round(w[j], h16);
+            skc0 += wh * c[j][0];
+            skc1 += wh * c[j][1];
+            skc2 += wh * c[j][2];
+            ska += wh;
             pnc = pnc * (1.0f - w[j]);
         }
     }
     if (A.tile_stats && lane == 0) A.tile_stats[4 * (size_t)(A.tile_base_launch + tile) + 2] = nrounds;
     if (!active) return;
-    store_pixel(A, V, pix, r, depth_on, T, pnc, skc0, skc1, skc2, ska, C0, C1, C2, D, ncomp, (uint32_t)nskip);
+    store_pixel(A, V, pix, r, depth_on, T, Tf, pnc, skc0, skc1, skc2, ska, C0, C1, C2, D, ncomp, (uint32_t)nskip);
 }
 
 // ---------------------------------------------------------------------------
@@ -1159,10 +1181,14 @@ struct GradTable {
 };
 
 // Gradient of one composited hit; the terms go to sink(gid, terms).
+// Ti is the transmittance before the hit; with rebuild set it is instead
+// Tafter / (1 - alpha) from the recomputed fp32 alpha and Tafter becomes Ti:
+// the reference's reconstruction from the final transmittance
+// (backward_pass.cu), used when the forward composited fp16 alphas.
 template <typename Sink>
 __device__ __forceinline__ void backward_hit(const TraceArgs &A, const DevView &V, const LaneRay &r, float p,
                                              float eps_sg, const PixelState &ps, uint32_t gid, float Ti, float &gT,
-                                             Sink &&sink) {
+                                             Sink &&sink, bool rebuild, float &Tafter) {
 #pragma clang fp contract(off)
     const GeoRecord rec = A.geo[gid];
     float dist, hx, hy, hz;
@@ -1170,6 +1196,10 @@ __device__ __forceinline__ void backward_hit(const TraceArgs &A, const DevView &
         return;
     float gv, q;
     const float a = gray_alpha(hx, hy, hz, rec.clip, rec.op, p, gv, q);
+    if (rebuild) {
+        Ti = Tafter / (1.0f - a);
+        Tafter = Ti;
+    }
     const float c0 = V.rgb[3 * gid + 0], c1 = V.rgb[3 * gid + 1], c2 = V.rgb[3 * gid + 2];
     const float w = Ti * a;
     float t[kGradTerms];
@@ -1299,10 +1329,11 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored(TraceArgs A
         const int R = V.ray_count;
         float gT = 0.0f;
         auto sink = [&](uint32_t gid, const float *t) { tab.add(A, V, gid, t); };
+        float Tafter = ps.Tf;  // fp16_hits: transmittances rebuilt from the final one
         for (int k = n - 1; k >= 0; --k) {
             const uint32_t gid = V.hit_ids[(size_t)k * R + pix];
             const float Ti = V.hit_T[(size_t)k * R + pix];
-            backward_hit(A, V, r, p, eps_sg, ps, gid, Ti, gT, sink);
+            backward_hit(A, V, r, p, eps_sg, ps, gid, Ti, gT, sink, A.fp16 != 0, Tafter);
         }
     }
     __syncthreads();
@@ -1337,10 +1368,20 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
     const float eps_sg = *A.knobs.eps_scale_grad;
     const float znear = *V.znear, zfar = *V.zfar;
     KBuf kb{s_kd, s_ki, s_ka, lane};
-    float Atot = 0.0f;
+    const bool h16 = A.fp16 != 0;
+    // The gradient sweep of a round needs the gradient flowing in from the
+    // hits behind it: (Atot - P) / T with Atot the colour the forward
+    // composited (against g) and P, T the part composited so far.  With fp32
+    // alphas the forward's sums serve.  With fp16_hits the reference rebuilds
+    // each hit's transmittance from the final one by fp32 alphas
+    // (backward_pass.cu), so a first replay forms the fp32-alpha totals: Q =
+    // prod(1 - a32) and Atot = sum g.c a32 T32, and the second replay scales
+    // T32 by Tfinal / Q.
+    float Atot = 0.0f, scale = 1.0f;
     if (need)
         Atot = (ps.g0 * V.comp_rgb[3 * pix + 0] + ps.g1 * V.comp_rgb[3 * pix + 1]) + ps.g2 * V.comp_rgb[3 * pix + 2];
-    float P = 0.0f, T = 1.0f, tmin = znear;
+    for (int pass = h16 ? 0 : 1; pass < 2; ++pass) {
+    float P = 0.0f, T = 1.0f, T32 = 1.0f, P32 = 0.0f, tmin = znear;
     int done = 0;
     bool alive = need && tmin < zfar;
     // same round cursor as the forward
@@ -1359,6 +1400,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
             float dist, hx, hy, hz;
             if (!gray_hit_test(g.m, g.mu[0], g.mu[1], g.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz))
                 return;
+            dist = fp16_round(dist, h16);
             const bool beyond = strict ? dist > cur_d : (dist > cur_d || (dist == cur_d && gid > cur_id));
             if (!beyond) return;
             float gv, q;
@@ -1380,7 +1422,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
             for (int k = 0; k < count; ++k) {
                 const float d = kb.dist(k);
                 const uint32_t gid = kb.gid(k);
-                const float a = kb.alpha(k);
+                const float a32 = kb.alpha(k);
+                const float a = fp16_round(a32, h16);  // as the forward composited it
                 if (terminated) {
                     kb.id[k * GRAY_WAVE + lane] = GRAY_EMPTY;
                     continue;
@@ -1388,10 +1431,12 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
                 tmin = fmaxf(d, tmin);
                 if (d < zfar) {
                     const float c0 = V.rgb[3 * gid + 0], c1 = V.rgb[3 * gid + 1], c2 = V.rgb[3 * gid + 2];
-                    const float w = T * a;
-                    P += ((ps.g0 * c0 + ps.g1 * c1) + ps.g2 * c2) * w;
-                    kb.a[k * GRAY_WAVE + lane] = T;  // T before this hit
+                    const float gdot = (ps.g0 * c0 + ps.g1 * c1) + ps.g2 * c2;
+                    P += gdot * (T * a);
+                    P32 += gdot * (T32 * a32);
+                    kb.a[k * GRAY_WAVE + lane] = h16 ? scale * T32 : T;  // transmittance before this hit
                     T = T * (1.0f - a);
+                    T32 = T32 * (1.0f - a32);
                     ++done;
                     used = k + 1;
                     if (T < tthr) {
@@ -1403,12 +1448,15 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
                 }
             }
             const bool last = done >= n;
-            float gT = last ? 0.0f : (Atot - P) / T;
-            for (int k = used - 1; k >= 0; --k) {
-                const uint32_t gid = kb.gid(k);
-                if (gid == GRAY_EMPTY) continue;
-                backward_hit(A, V, r, p, eps_sg, ps, gid, kb.alpha(k), gT,
-                             [&](uint32_t g, const float *t) { grad_to_global(A, V, g, t); });
+            if (pass == 1) {
+                float gT = last ? 0.0f : (h16 ? (Atot - P32) / T32 : (Atot - P) / T);
+                for (int k = used - 1; k >= 0; --k) {
+                    const uint32_t gid = kb.gid(k);
+                    if (gid == GRAY_EMPTY) continue;
+                    float unused = 0.0f;
+                    backward_hit(A, V, r, p, eps_sg, ps, gid, kb.alpha(k), gT,
+                                 [&](uint32_t g, const float *t) { grad_to_global(A, V, g, t); }, false, unused);
+                }
             }
             alive = false;
             if (!terminated && !last && count == GRAY_KB) {
@@ -1426,6 +1474,11 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
                 alive = tmin < zfar && batch < GRAY_MAX_ROUNDS;
             }
         }
+    }
+    if (pass == 0 && need) {
+        Atot = P32;
+        scale = T32 > 0.0f ? ps.Tf / T32 : 1.0f;
+    }
     }
 }
 
@@ -1548,6 +1601,7 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
     A.total_tiles = tiles;
     A.xcd_chunk = (tiles + 7) / 8;
     A.save = args->save;
+    A.fp16 = args->fp16_hits ? 1 : 0;
     A.capacity = args->hit_capacity < 0 ? 0 : args->hit_capacity;
     A.use_bvh = args->use_bvh;
     A.num_gaussians = (int)ctx->n;

@@ -47,6 +47,7 @@ class RenderSettings:
     sh_degree: int = 3
     memory_cap_bytes: int = 4 << 30
     list_capacity: Optional[int] = None   # None: from the memory cap
+    fp16_hits: bool = False               # the reference's fp16 hit storage (distances, compositing alphas)
     hit_capacity: Optional[int] = None    # None: from the memory cap
 
 
@@ -160,6 +161,7 @@ class MultiViewRenderer:
         a.views = C.cast(views, C.POINTER(L.View))
         a.save = 1 if save else 0
         a.hit_capacity, a.list_capacity = self.last_capacities
+        a.fp16_hits = 1 if self.settings.fp16_hits else 0
         a.use_bvh = 1
         return a
 

@@ -319,6 +319,9 @@ class NativeRaytracer:
         # traversing); rays with more hits are re-traversed per batch
         self.list_capacity = DEFAULT_LIST_CAPACITY
         self.collect_budget = 0  # walk iterations before work is split into tasks (0: built-in)
+        # the reference's fp16 storage of hit distances and alphas
+        # (per_pixel_linked_list.h:28-29); False: fp32 throughout
+        self.fp16_hits = False
         self.tile_stats = None  # optional int32 [tiles, 4] traversal statistics of the forward
         self.timer = _NoTimer()
         self._steps = 0
@@ -378,6 +381,7 @@ class NativeRaytracer:
         a.tile_stats = self.tile_stats.data_ptr() if self.tile_stats is not None else None
         a.list_capacity = int(self.list_capacity)
         a.collect_budget = int(self.collect_budget)
+        a.fp16_hits = 1 if self.fp16_hits else 0
         return a
 
     def _require_bvh(self) -> None:

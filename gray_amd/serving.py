@@ -154,6 +154,7 @@ class SceneServer:
             a.num_views = V
             a.views = C.cast(views, C.POINTER(L.View))
             a.save, a.hit_capacity, a.use_bvh, a.list_capacity = 0, 0, 1, self.list_capacity
+            a.fp16_hits = 1 if s.fp16_hits else 0
             L.check(L.LIB.gray_render_forward(slot.ctx.handle, C.byref(a), sp))
             slot.end = torch.cuda.Event(enable_timing=True)
             slot.end.record(slot.stream)

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 4
+#define GRAY_ABI_VERSION 5
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -128,6 +128,11 @@ typedef struct {
     int32_t collect_budget;     /* iterations a ray's walk runs before its remaining subtrees become tasks
                                    that other lanes take (load balance of rays that cross many boxes);
                                    <= 0 = the built-in default */
+    int32_t fp16_hits;          /* 1 = the reference's hit storage numerics (per_pixel_linked_list.h:28-29):
+                                   hit distances and the alphas used for compositing, depth and the
+                                   truncation estimate rounded to fp16 (round to nearest even); the full
+                                   transmittance keeps the fp32 alpha (shaders.cu).  0 = fp32 throughout */
+    int32_t pad;
 } gray_render_args;
 
 /* Gradient targets of the backward (reference gaussians.h:34-62). */

@@ -45,7 +45,37 @@ typedef struct {
     float bg[3];
     float znear, zfar;
     int render_depth;
+    int fp16_hits; /* reference hit storage: distances and compositing alphas rounded to fp16 */
 } or_knobs;
+
+/* float -> IEEE binary16 -> float, round to nearest even (the reference's
+ * __float2half_rn, per_pixel_linked_list.h:28-29), written out on the bits. */
+float or_fp16_round(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    const uint32_t sign = u & 0x80000000u;
+    uint32_t a = u & 0x7fffffffu;
+    if (a >= 0x7f800000u) return x;      /* inf, nan */
+    if (a >= 0x477ff000u) {              /* >= 65520 rounds past 65504 */
+        u = sign | 0x7f800000u;
+        memcpy(&x, &u, 4);
+        return x;
+    }
+    if (a < 0x38800000u) {               /* below 2^-14: binary16 subnormals, spacing 2^-24 */
+        const float q = 5.9604644775390625e-08f;
+        const float r = nearbyintf(fabsf(x) / q) * q; /* exact scaling; nearbyint rounds half to even */
+        return sign ? -r : r;
+    }
+    a += 0x0fffu + ((a >> 13) & 1u);     /* drop 13 mantissa bits, ties to even */
+    a &= 0xffffe000u;
+    u = sign | a;
+    memcpy(&x, &u, 4);
+    return x;
+}
+
+void or_fp16_round_array(int64_t n, const float *x, float *y) {
+    for (int64_t i = 0; i < n; ++i) y[i] = or_fp16_round(x[i]);
+}
 
 static float sigmoidf_(float x) { return 1.0f / (1.0f + expf(-x)); }
 
@@ -177,7 +207,7 @@ static int hit_cmp(const void *x, const void *y) {
 
 /* every hit of one ray, in Gaussian-id order; returns count (<= cap written) */
 static int collect(int n, const float *geo, float athr, float p, const float *o, const float *d, hitrec *out,
-                   int cap) {
+                   int cap, int fp16) {
     int cnt = 0;
     for (int g = 0; g < n; ++g) {
         const float *G = geo + 16 * (size_t)g;
@@ -185,7 +215,7 @@ static int collect(int n, const float *geo, float athr, float p, const float *o,
         float dist, h[3];
         if (!or_hit(G, o, d, &dist, h)) continue;
         if (cnt < cap) {
-            out[cnt].d = dist;
+            out[cnt].d = fp16 ? or_fp16_round(dist) : dist;
             out[cnt].id = (uint32_t)g;
             out[cnt].a = or_alpha(G, h, p, NULL, NULL);
         }
@@ -202,7 +232,7 @@ void or_hitlists(int n, const float *geo, float athr, int R, const float *ro, co
         hitrec *buf = (hitrec *)malloc(sizeof(hitrec) * (size_t)(n > 0 ? n : 1));
 #pragma omp for schedule(dynamic, 16)
         for (int r = 0; r < R; ++r) {
-            const int cnt = collect(n, geo, athr, 2.0f, ro + 3 * (size_t)r, rd + 3 * (size_t)r, buf, n);
+            const int cnt = collect(n, geo, athr, 2.0f, ro + 3 * (size_t)r, rd + 3 * (size_t)r, buf, n, 0);
             qsort(buf, (size_t)cnt, sizeof(hitrec), hit_cmp);
             counts[r] = cnt;
             for (int k = 0; k < cnt && k < maxh; ++k) {
@@ -231,21 +261,25 @@ void or_render(int n, const float *geo, const float *rgb, int R, const float *ro
         for (int r = 0; r < R; ++r) {
             const float *o = ro + 3 * (size_t)r, *d = rd + 3 * (size_t)r;
             const float dl = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
-            const int cnt = dl != 0.0f ? collect(n, geo, athr, p, o, d, buf, n) : 0;
+            const int h16 = k->fp16_hits;
+            const int cnt = dl != 0.0f ? collect(n, geo, athr, p, o, d, buf, n, h16) : 0;
             if (nhits) nhits[r] = cnt;
-            /* literal reference sums over all hits (traversal order = id order here) */
+            /* literal reference sums over all hits (traversal order = id order here): the
+             * full transmittance of the hit program (fp32 alpha), the skipped sums of the
+             * stored (fp16 in fp16_hits mode) alphas */
             float full_T = 1.0f, sk0 = 0.0f, sk1 = 0.0f, sk2 = 0.0f, ska = 0.0f;
             for (int i = 0; i < cnt; ++i) {
                 full_T *= 1.0f - buf[i].a;
+                const float ah = h16 ? or_fp16_round(buf[i].a) : buf[i].a;
                 const float *c = rgb + 3 * (size_t)buf[i].id;
-                sk0 += buf[i].a * c[0];
-                sk1 += buf[i].a * c[1];
-                sk2 += buf[i].a * c[2];
-                ska += buf[i].a;
+                sk0 += ah * c[0];
+                sk1 += ah * c[1];
+                sk2 += ah * c[2];
+                ska += ah;
             }
             qsort(buf, (size_t)cnt, sizeof(hitrec), hit_cmp);
             memset(comp, 0, (size_t)(cnt > 0 ? cnt : 1));
-            float T = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, depth = 0.0f, tmin = k->znear;
+            float T = 1.0f, Tf = 1.0f, c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, depth = 0.0f, tmin = k->znear;
             int nc = 0;
             int start = 0; /* first sorted index with dist > tmin */
             for (int it = 0; it < MAX_ROUNDS && tmin < k->zfar; ++it) {
@@ -257,7 +291,8 @@ void or_render(int n, const float *geo, const float *rgb, int R, const float *ro
                     const float dist = buf[i].d;
                     tmin = dist > tmin ? dist : tmin;
                     if (dist < k->zfar) {
-                        const float a = buf[i].a;
+                        const float a32 = buf[i].a;
+                        const float a = h16 ? or_fp16_round(a32) : a32;
                         const float *c = rgb + 3 * (size_t)buf[i].id;
                         if (mode == 1) {
                             sk0 -= a * c[0];
@@ -275,6 +310,7 @@ void or_render(int n, const float *geo, const float *rgb, int R, const float *ro
                             list_T[(size_t)r * cap + nc] = T;
                         }
                         T = T * (1.0f - a);
+                        Tf = Tf * (1.0f - a32);
                         comp[i] = 1;
                         ++nc;
                         if (T < tthr) {
@@ -300,15 +336,16 @@ void or_render(int n, const float *geo, const float *rgb, int R, const float *ro
                 float pnc = 1.0f, s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f;
                 for (int i = 0; i < cnt; ++i) {
                     if (comp[i]) continue;
-                    const float a = buf[i].a;
+                    const float a32 = buf[i].a;
+                    const float a = h16 ? or_fp16_round(a32) : a32;
                     const float *c = rgb + 3 * (size_t)buf[i].id;
                     s0 += a * c[0];
                     s1 += a * c[1];
                     s2 += a * c[2];
                     sa += a;
-                    pnc = pnc * (1.0f - a);
+                    pnc = pnc * (1.0f - a32);
                 }
-                F = T * pnc;
+                F = Tf * pnc;
                 if (sa > 0.0f) {
                     e0 = s0 / sa;
                     e1 = s1 / sa;

@@ -25,19 +25,30 @@ class Knobs(C.Structure):
     _fields_ = [
         ("exp_power", C.c_float), ("alpha_threshold", C.c_float), ("t_threshold", C.c_float),
         ("eps_fwd", C.c_float), ("eps_scale_grad", C.c_float), ("bg", C.c_float * 3),
-        ("znear", C.c_float), ("zfar", C.c_float), ("render_depth", C.c_int),
+        ("znear", C.c_float), ("zfar", C.c_float), ("render_depth", C.c_int), ("fp16_hits", C.c_int),
     ]
 
 
 def make_knobs(exp_power=2.0, alpha_threshold=0.01, t_threshold=0.01, eps_fwd=1e-12, eps_scale_grad=1e-12,
-               bg=(0.0, 0.0, 0.0), znear=0.0, zfar=99999.9, render_depth=False) -> Knobs:
+               bg=(0.0, 0.0, 0.0), znear=0.0, zfar=99999.9, render_depth=False, fp16_hits=False) -> Knobs:
     k = Knobs()
     k.exp_power, k.alpha_threshold, k.t_threshold = exp_power, alpha_threshold, t_threshold
     k.eps_fwd, k.eps_scale_grad = eps_fwd, eps_scale_grad
     for i in range(3):
         k.bg[i] = bg[i]
     k.znear, k.zfar, k.render_depth = znear, zfar, int(bool(render_depth))
+    k.fp16_hits = int(bool(fp16_hits))
     return k
+
+
+def fp16_round(x) -> np.ndarray:
+    """float -> binary16 (round to nearest even) -> float, as the reference's
+    __float2half_rn storage of hit distances and alphas
+    (per_pixel_linked_list.h:28-29)."""
+    x = _f(x)
+    y = np.empty_like(x)
+    lib().or_fp16_round_array(C.c_int64(x.size), _p(x), _p(y))
+    return y
 
 
 def build() -> str:

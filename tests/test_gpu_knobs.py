@@ -62,3 +62,49 @@ def test_knobs_forward_backward(case, lists):
     for name, key in (("grad_opacity", "opacity"), ("grad_scale", "scale"), ("grad_rotation", "rotation"),
                       ("grad_channels", "rgb")):
         assert rel_err(getattr(g, name).cpu(), ob[key]) < 1e-3, name
+
+
+@pytest.mark.parametrize("lists,hit_cap", [(None, None), (0, None), (3, None), (None, 3)],
+                         ids=["lists", "traverse", "mixed", "recompute-backward"])
+def test_fp16_hits_forward_backward(lists, hit_cap):
+    """The reference's fp16 hit storage: distances and compositing alphas
+    rounded to fp16, full transmittance from fp32 alphas."""
+    scene = scenes.tiny_scene(1024, seed=31)
+    cam = scenes.tiny_camera(48, 40)
+    bg = (0.2, 0.3, 0.1)
+    imgs = {}
+    for fp16 in (False, True):
+        nr = setup_native(scene, cam, t_threshold=0.02, bg=bg, list_capacity=lists, hit_capacity=hit_cap)
+        nr.fp16_hits = fp16
+        nr.get_config().render_depth.fill_(True)
+        torch.set_grad_enabled(True)
+        nr.forward_pass()
+        fb = nr.get_framebuffer()
+        imgs[fp16] = fb.output_channels[:40, :48].reshape(-1, 3).cpu().numpy()
+    fb.grad_output_channels.copy_(torch.randn(fb.grad_output_channels.shape,
+                                              generator=torch.Generator().manual_seed(3)).cuda())
+    g = nr.get_gaussians()
+    for name in ("grad_mean", "grad_rotation", "grad_scale", "grad_opacity", "grad_channels"):
+        getattr(g, name).zero_()
+    nr.backward_pass()
+    torch.cuda.synchronize()
+    nr.device_status()
+    assert np.abs(imgs[True] - imgs[False]).max() > 1e-6, "the mode must change the image"
+    geo, _ = nr.debug_snapshot()
+    geo = geo.cpu().numpy()
+    ro, rd = rays_of(nr)
+    rgb = g.channels.cpu().numpy()
+    k = O.make_knobs(t_threshold=0.02, bg=bg, render_depth=True, fp16_hits=True)
+    ref = O.render(geo, rgb, ro, rd, k)
+    assert np.abs(imgs[True] - ref["C"]).max() < 1e-5
+    depth = fb.output_depth[:40, :48].reshape(-1).cpu().numpy()
+    assert np.abs(depth - ref["D"]).max() < 1e-4 * max(1.0, np.abs(ref["D"]).max())
+    T = fb.transmittance[:40, :48].reshape(-1).cpu().numpy()
+    F = fb.full_transmittance[:40, :48].reshape(-1).cpu().numpy()
+    assert np.abs(T - ref["T"]).max() < 1e-5 and np.abs(F - ref["F"]).max() < 1e-5
+    _, l2w = O.snapshot(scene.mean, scene.rotation, scene.scale, scene.opacity, 0.01, 2.0)
+    gout = fb.grad_output_channels[:40, :48].reshape(-1, 3).cpu().numpy()
+    ob = O.backward(geo, l2w, rgb, scene.rotation, scene.scale, scene.opacity, ro, rd, ref, gout, k, 48 * 40)
+    for name, key in (("grad_opacity", "opacity"), ("grad_scale", "scale"), ("grad_rotation", "rotation"),
+                      ("grad_channels", "rgb")):
+        assert rel_err(getattr(g, name).cpu(), ob[key]) < 1e-3, name

@@ -371,3 +371,68 @@ def test_finite_differences_of_reference_formulation():
         assert abs(fd - an) < 1e-5 * max(1.0, abs(an)), (pi, idx, fd, an)
         checked += 1
     assert checked >= 10
+
+
+# ---------------------------------------------------------------------------
+# fp16 hit storage (the reference keeps distances and alphas as fp16,
+# per_pixel_linked_list.h:28-29)
+# ---------------------------------------------------------------------------
+def test_fp16_round_matches_numpy(oracle_lib):
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal(100000) * 10.0 ** rng.uniform(-9, 5, 100000)).astype(np.float32)
+    edges = np.array([65504, 65519.99, 65520, 65535, 1e5, -65520, 6.1035156e-05, 6.1e-5, 5.96e-8, 2.98e-8,
+                      2.99e-8, 1e-9, 0.0, -0.0, np.inf, -np.inf, 1 + 2**-11, 1 + 3 * 2**-11, 2049, 2051, 4097],
+                     np.float32)
+    x = np.concatenate([x, edges])
+    with np.errstate(over="ignore"):
+        ref = x.astype(np.float16).astype(np.float32)
+    assert np.array_equal(O.fp16_round(x).view(np.uint32), ref.view(np.uint32))
+
+
+def test_fp16_hits_single_gaussian_closed_form(oracle_lib):
+    # one hit: compositing uses the fp16 alpha (T = 1 - a16), the full
+    # transmittance the fp32 one (F = 1 - a32, unchanged by the mode), the skip
+    # set is empty, so C = a16 c + bg F; depth is the fp16 distance
+    mean = np.array([[-0.05, -0.05, 0.0]])
+    log_scale = np.log([[0.1, 0.3, 0.3]])
+    colors = np.array([[0.7, 0.2, 0.4]], np.float32)
+    rot = np.array([[-0.5, 0.0, 0.0, 1.0]])
+    logit = np.array([[0.3]])
+    bg = (0.1, 0.2, 0.3)
+    ro, rd = mock_camera_rays(oracle_lib)
+    geo, _ = O.snapshot(mean, rot, log_scale, logit, 0.01, 2.0)
+    f32 = O.render(geo, colors, ro, rd, O.make_knobs(t_threshold=0.03, bg=bg, render_depth=True))
+    f16 = O.render(geo, colors, ro, rd, O.make_knobs(t_threshold=0.03, bg=bg, render_depth=True, fp16_hits=True))
+    hit = f32["ncomp"] > 0
+    assert hit.sum() > 20 and np.array_equal(f16["ncomp"], f32["ncomp"])
+    assert np.array_equal(f16["F"], f32["F"])
+    a32 = 1.0 - f32["F"][hit].astype(np.float64)
+    a16 = 1.0 - f16["T"][hit].astype(np.float64)
+    assert np.abs(a16 - O.fp16_round(a32)).max() < 2e-7
+    assert (f16["T"][hit] != f32["T"][hit]).any(), "the rounding must change the composited alpha"
+    expect = a16[:, None] * colors[0] + np.array(bg) * f32["F"][hit][:, None]
+    assert np.abs(f16["C"][hit] - expect).max() < 1e-6
+    assert np.abs(f16["D"][hit] - O.fp16_round(f32["D"][hit])).max() <= 1e-6 * np.abs(f32["D"][hit]).max()
+    assert np.array_equal(f16["C"][~hit], f32["C"][~hit])
+
+
+def test_fp16_hits_literal_and_skip_set_agree(oracle_lib):
+    # the skip-set formulation and the reference's literal one (all-hit sums
+    # minus composited, forward_pass.cu:53-57,85-88) agree in fp16 mode too
+    rng = np.random.default_rng(5)
+    n = 300
+    mean = rng.normal(0, 0.4, (n, 3))
+    log_scale = rng.normal(-2.5, 0.4, (n, 3))
+    rot = rng.normal(size=(n, 4))
+    logit = rng.normal(size=(n, 1))
+    colors = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    ro, rd = mock_camera_rays(oracle_lib)
+    geo, _ = O.snapshot(mean, rot, log_scale, logit, 0.01, 2.0)
+    k = O.make_knobs(t_threshold=0.2, bg=(0.3, 0.1, 0.2), fp16_hits=True)
+    a = O.render(geo, colors, ro, rd, k, mode=0)
+    b = O.render(geo, colors, ro, rd, k, mode=1)
+    assert (a["nhits"] > a["ncomp"]).sum() > 50
+    assert np.abs(a["C"] - b["C"]).max() < 1e-5
+    assert np.abs(a["F"] - b["F"]).max() < 1e-5
+    f32 = O.render(geo, colors, ro, rd, O.make_knobs(t_threshold=0.2, bg=(0.3, 0.1, 0.2)), mode=0)
+    assert np.abs(a["C"] - f32["C"]).max() > 1e-5
