@@ -99,6 +99,24 @@ def make_step(rt, cam, target, loss="l1"):
     return step_l1ssim if loss == "l1ssim" else step
 
 
+def cpu_adam_ms(n: int, steps: int = 3) -> float:
+    """torch.optim.Adam(foreach=True) on the host over the full parameter
+    set of n Gaussians (59 floats each: mean 3, rotation 4, scale 3, opacity
+    1, SH 48), the CPU counterpart of the fused Adam launch; ms per step."""
+    import torch
+
+    shapes = [(n, 3), (n, 4), (n, 3), (n, 1), (n, 1, 3), (n, 15, 3)]
+    params = [torch.zeros(s, requires_grad=True) for s in shapes]
+    for q in params:
+        q.grad = torch.full(q.shape, 1e-3)
+    opt = torch.optim.Adam(params, lr=1e-3, eps=1e-15, foreach=True)
+    opt.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        opt.step()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
 def cpu_baseline(scene, cam, cfg, sample_rays: int, seed: int = 0):
     """Forward + backward of the CPU oracle over a seeded sample of pixels
     against all N Gaussians; returns rays/s on the host cores."""
@@ -319,6 +337,10 @@ def run_train(args, traffic):
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline on {args.cpu_rays} rays")
         out["cpu_baseline"] = cpu_baseline(scene, cam, cfg, args.cpu_rays)
+        import torch
+
+        out["cpu_baseline"]["adam_step_ms"] = round(cpu_adam_ms(n), 1)
+        out["cpu_baseline"]["adam_threads"] = torch.get_num_threads()
     else:
         out["cpu_baseline"] = None
     return out

@@ -120,6 +120,7 @@ def _load():
         "gray_debug_list_counts": (i32, [vp, vp, i64, C.POINTER(i64), vp]),
         "gray_render_stage_ms": (i32, [vp, C.POINTER(C.c_float), i32]),
         "gray_render_reserve": (i32, [vp, i64, i32]),
+        "gray_render_ellipsoids": (i32, [vp, C.POINTER(RenderArgs), vp, vp]),
         "gray_loss_workspace_bytes": (i32, [i32, i32, i32, C.POINTER(i64)]),
         "gray_loss_forward": (i32, [C.POINTER(LossArgs), vp]),
         "gray_loss_backward": (i32, [C.POINTER(LossArgs), vp]),
@@ -142,6 +143,7 @@ EXPORTED = (
     "gray_sh_backward", "gray_render_forward", "gray_render_backward", "gray_adam_step", "gray_debug_hitlists",
     "gray_debug_snapshot", "gray_debug_bvh", "gray_debug_list_counts", "gray_render_stage_ms",
     "gray_render_reserve", "gray_loss_workspace_bytes", "gray_loss_forward", "gray_loss_backward",
+    "gray_render_ellipsoids",
 )
 
 

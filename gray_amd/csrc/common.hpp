@@ -134,6 +134,44 @@ __device__ __forceinline__ bool gray_hit_test(const float *__restrict__ m, float
     return true;
 }
 
+// Display test of the proxy ellipsoid (the unit sphere in local coordinates),
+// reference ellipsoid_viewer.cu:21-66: stable closest-approach form, reject
+// when the closest approach lies behind the origin, the entry point if it lies
+// in [tlo, thi] else the exit point if it does.  t is world-space; n is the
+// (unnormalised) local surface point.
+__device__ __forceinline__ bool gray_ellipsoid_hit(const float *__restrict__ m, float mux, float muy, float muz,
+                                                   float ox, float oy, float oz, float dx, float dy, float dz,
+                                                   float tlo, float thi, float &t, float &nx, float &ny, float &nz) {
+#pragma clang fp contract(off)
+    const float px = ox - mux, py = oy - muy, pz = oz - muz;
+    const float lox = (m[0] * px + m[1] * py) + m[2] * pz;
+    const float loy = (m[3] * px + m[4] * py) + m[5] * pz;
+    const float loz = (m[6] * px + m[7] * py) + m[8] * pz;
+    const float ldx = (m[0] * dx + m[1] * dy) + m[2] * dz;
+    const float ldy = (m[3] * dx + m[4] * dy) + m[5] * dz;
+    const float ldz = (m[6] * dx + m[7] * dy) + m[8] * dz;
+    if ((lox * ldx + loy * ldy) + loz * ldz > 0.0f) return false;
+    const float norm = sqrtf((ldx * ldx + ldy * ldy) + ldz * ldz);
+    const float ux = ldx / norm, uy = ldy / norm, uz = ldz / norm;
+    const float smid = -((lox * ux + loy * uy) + loz * uz);
+    const float cx = lox + smid * ux, cy = loy + smid * uy, cz = loz + smid * uz;
+    const float sq = (cx * cx + cy * cy) + cz * cz;
+    if (sq > 1.0f) return false;
+    const float half = sqrtf(1.0f - sq);
+    const float t0 = (smid - half) / norm, t1 = (smid + half) / norm;
+    if (t0 >= tlo && t0 <= thi)
+        t = t0;
+    else if (t1 >= tlo && t1 <= thi)
+        t = t1;
+    else
+        return false;
+    const float tn = t * norm;
+    nx = lox + tn * ux;
+    ny = loy + tn * uy;
+    nz = loz + tn * uz;
+    return true;
+}
+
 // Generalised Gaussian kernel (reference kernel.cu:24-38) evaluated at the
 // local hit scaled by the clip factor; returns alpha (kernel.cu:40-42).
 // q_out = |x*clip|^2 and g_out = kernel value are returned for the backward.

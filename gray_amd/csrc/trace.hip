@@ -1516,6 +1516,70 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_debug_hits(TraceArgs A, const Geo
     if (inside) A.dbg_count[ray] = active ? cnt : 0;
 }
 
+// Ellipsoid display mode (reference raygen_ellipsoids + __intersection__is_ellipsoid
+// + __closesthit__ellipsoid, ellipsoid_viewer.cu:1-94, selected at
+// shaders.cu:90-93): the nearest proxy ellipsoid of opacity >= min_op in
+// [znear, zfar] (ties: lower id), flat-shaded max(0.5, -d.n) (colour + 0.15),
+// depth = hit distance; background and depth 0 on a miss.  Rays with a zero
+// direction leave their pixel untouched.
+__global__ __launch_bounds__(GRAY_WAVE) void k_ellipsoids(TraceArgs A, const GeoRecord *__restrict__ geo,
+                                                          const BvhNode *__restrict__ nodes,
+                                                          const float *__restrict__ min_op) {
+    __shared__ uint32_t s_stack[GRAY_STACK * GRAY_WAVE];
+    int vi, tx, ty;
+    if (!block_tile(A, vi, tx, ty)) return;
+    const DevView &V = A.views[vi];
+    const int lane = threadIdx.x;
+    const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
+    const bool inside = x < V.width && y < V.height;
+    LaneRay r{};
+    bool active = false;
+    if (inside) active = make_ray(V, x, y, r);
+    const float znear = *V.znear, zfar = *V.zfar, mo = *min_op;
+    float best = INFINITY, bnx = 0.0f, bny = 0.0f, bnz = 0.0f;
+    uint32_t bid = GRAY_EMPTY;
+    auto leaf = [&](uint32_t gid, const GeoV &g, float, float, float) {
+        if (!(g.op > 0.0f) || g.op < mo) return;  // not a proxy, or below the display threshold
+        float t, nx, ny, nz;
+        if (!gray_ellipsoid_hit(g.m, g.mu[0], g.mu[1], g.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, znear, zfar, t,
+                                nx, ny, nz))
+            return;
+        if (t < best || (t == best && gid < bid)) {
+            best = t;
+            bid = gid;
+            bnx = nx;
+            bny = ny;
+            bnz = nz;
+        }
+    };
+    traverse(A, nodes, geo, V.rgb, false, r, active, 0.0f, s_stack, lane, leaf);
+    if (!active) return;
+    const int pix = y * V.stride + x;
+    const float *bg = A.knobs.bg;
+    float c0 = bg[0], c1 = bg[1], c2 = bg[2], depth = 0.0f;
+    if (bid != GRAY_EMPTY) {
+#pragma clang fp contract(off)
+        const GeoV g = load_geo(geo, bid);
+        // local normal -> world: the transpose of world-to-local
+        const float ln = sqrtf((bnx * bnx + bny * bny) + bnz * bnz);
+        const float lx = bnx / ln, ly = bny / ln, lz = bnz / ln;
+        const float wx = (g.m[0] * lx + g.m[3] * ly) + g.m[6] * lz;
+        const float wy = (g.m[1] * lx + g.m[4] * ly) + g.m[7] * lz;
+        const float wz = (g.m[2] * lx + g.m[5] * ly) + g.m[8] * lz;
+        const float wn = sqrtf((wx * wx + wy * wy) + wz * wz);
+        const float dn = sqrtf((r.dx * r.dx + r.dy * r.dy) + r.dz * r.dz);
+        const float align = fmaxf(0.5f, ((-r.dx / dn) * (wx / wn) + (-r.dy / dn) * (wy / wn)) + (-r.dz / dn) * (wz / wn));
+        c0 = align * (V.rgb[3 * bid + 0] + 0.15f);
+        c1 = align * (V.rgb[3 * bid + 1] + 0.15f);
+        c2 = align * (V.rgb[3 * bid + 2] + 0.15f);
+        depth = best;
+    }
+    V.out_c[3 * pix + 0] = c0;
+    V.out_c[3 * pix + 1] = c1;
+    V.out_c[3 * pix + 2] = c2;
+    if (V.out_depth) V.out_depth[pix] = depth;
+}
+
 __global__ void k_debug_sort(int rays, int maxh, uint32_t *__restrict__ ids, float *__restrict__ dist,
                              const int32_t *__restrict__ counts) {
     const int ray = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1848,6 +1912,32 @@ extern "C" int gray_render_backward(gray_ctx *ctx, const gray_render_args *args,
         }
         hipLaunchKernelGGL(k_trace_backward_recompute, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, A.geo,
                            A.nodes);
+        GRAY_HIP(hipGetLastError());
+    }
+    return GRAY_OK;
+}
+
+extern "C" int gray_render_ellipsoids(gray_ctx *ctx, const gray_render_args *args, const float *min_opacity,
+                                      void *stream) {
+    int rc = check_render(ctx, args);
+    if (rc) return rc;
+    if (!min_opacity) {
+        gray_set_error(GRAY_ERR_INVALID, "render_ellipsoids: null min_opacity");
+        return GRAY_ERR_INVALID;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    for (int first = 0; first < args->num_views; first += GRAY_MAX_VIEWS) {
+        const int nv = std::min(GRAY_MAX_VIEWS, args->num_views - first);
+        TraceArgs A;
+        if ((rc = fill_args(ctx, args, first, nv, A))) return rc;
+        for (int i = 0; i < nv; ++i) {
+            if (!A.views[i].out_c || !A.views[i].rgb) {
+                gray_set_error(GRAY_ERR_INVALID, "render_ellipsoids: view %d needs rgb and out_channels", first + i);
+                return GRAY_ERR_INVALID;
+            }
+        }
+        if (A.total_tiles == 0) continue;
+        hipLaunchKernelGGL(k_ellipsoids, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, A.geo, A.nodes, min_opacity);
         GRAY_HIP(hipGetLastError());
     }
     return GRAY_OK;

@@ -405,6 +405,10 @@ class NativeRaytracer:
         save = torch.is_grad_enabled() and not self.inference_only
         self.gaussian_data.was_visible.fill_(False)
         self._sh_forward()
+        if not save and self._ellipsoids_on():
+            self._render_ellipsoids()
+            self._forward_saved = False
+            return
         view = self._view(save)
         tk = self.timer.start("trace_forward")
         L.check(L.LIB.gray_render_forward(self.ctx.handle, C.byref(self._args(view, save)), self._stream()))
@@ -415,8 +419,23 @@ class NativeRaytracer:
         """Reference raytracer.h:104-114 (no metadata update, no backward state)."""
         self._require_bvh()
         self._sh_forward()
+        if self._ellipsoids_on():
+            self._render_ellipsoids()
+            return
         view = self._view(False)
         L.check(L.LIB.gray_render_forward(self.ctx.handle, C.byref(self._args(view, False)), self._stream()))
+
+    def _ellipsoids_on(self) -> bool:
+        """config.render_ellipsoids, read on the host only for renders without
+        gradients (the reference's display switch, shaders.cu:90-93; its
+        Python layer rejects it with gradients, gray/raytracer.py:142-144)."""
+        return bool(self.config_data.render_ellipsoids.item())
+
+    def _render_ellipsoids(self) -> None:
+        """Reference ellipsoid_viewer.cu: nearest proxy ellipsoid, flat shaded."""
+        view = self._view(False)
+        L.check(L.LIB.gray_render_ellipsoids(self.ctx.handle, C.byref(self._args(view, False)),
+                                             self.config_data.ellipsoid_min_opacity.data_ptr(), self._stream()))
 
     def backward_pass(self) -> None:
         """Reference raytracer.h:116-124."""

@@ -267,6 +267,16 @@ int gray_sh_backward(gray_ctx *ctx, const gray_params *params, const gray_knobs 
  * compositing in rounds of 32 with the reference truncation estimate. */
 int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, void *stream);
 
+/* Replaces the ellipsoid display mode of the forward launch (config
+ * render_ellipsoids, shaders.cu:90-93; ellipsoid_viewer.cu:1-94): per ray the
+ * nearest proxy ellipsoid (the clipped unit sphere of the geometry snapshot)
+ * with opacity >= *min_opacity (device [1]) in [znear, zfar], ties to the lower
+ * id; out_channels = max(0.5, -d.n) (rgb + 0.15) with n the world normal,
+ * out_depth = hit distance; background and depth 0 on a miss.  Needs each
+ * view's rgb and out_channels; out_depth may be NULL.  Nothing else is
+ * written. */
+int gray_render_ellipsoids(gray_ctx *ctx, const gray_render_args *args, const float *min_opacity, void *stream);
+
 /* Replaces Raytracer::backward_pass (raytracer.h:116-124, backward_pass.cu:2-142):
  * replays each ray's composited hits (stored by the forward, or recomputed by
  * re-traversal for rays whose list exceeded hit_capacity) back to front and

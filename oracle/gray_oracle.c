@@ -245,6 +245,71 @@ void or_hitlists(int n, const float *geo, float athr, int R, const float *ro, co
 }
 
 /*
+ * Ellipsoid display mode (ellipsoid_viewer.cu:1-94): per ray the nearest proxy
+ * ellipsoid with opacity >= min_op in [znear, zfar] (ties: lower id), the
+ * stable closest-approach sphere test in local coordinates, flat shading
+ * max(0.5, -d.n) (rgb + 0.15) with the world normal W2L^T n_local; depth = t.
+ * A miss leaves bg and depth 0.  Brute force over all Gaussians.
+ */
+void or_ellipsoids(int n, const float *geo, const float *rgb, int R, const float *ro, const float *rd, float min_op,
+                   float znear, float zfar, const float *bg, float *C, float *D) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int r = 0; r < R; ++r) {
+        const float *o = ro + 3 * (size_t)r, *d = rd + 3 * (size_t)r;
+        float best = INFINITY, bn[3] = {0.0f, 0.0f, 0.0f};
+        int bid = -1;
+        for (int g = 0; g < n; ++g) {
+            const float *G = geo + 16 * (size_t)g;
+            if (!(G[12] > 0.0f) || G[12] < min_op) continue;
+            const float p[3] = {o[0] - G[9], o[1] - G[10], o[2] - G[11]};
+            float lo[3], ld[3];
+            for (int i = 0; i < 3; ++i) {
+                lo[i] = (G[3 * i] * p[0] + G[3 * i + 1] * p[1]) + G[3 * i + 2] * p[2];
+                ld[i] = (G[3 * i] * d[0] + G[3 * i + 1] * d[1]) + G[3 * i + 2] * d[2];
+            }
+            if ((lo[0] * ld[0] + lo[1] * ld[1]) + lo[2] * ld[2] > 0.0f) continue;
+            const float norm = sqrtf((ld[0] * ld[0] + ld[1] * ld[1]) + ld[2] * ld[2]);
+            const float u[3] = {ld[0] / norm, ld[1] / norm, ld[2] / norm};
+            const float smid = -((lo[0] * u[0] + lo[1] * u[1]) + lo[2] * u[2]);
+            const float c[3] = {lo[0] + smid * u[0], lo[1] + smid * u[1], lo[2] + smid * u[2]};
+            const float sq = (c[0] * c[0] + c[1] * c[1]) + c[2] * c[2];
+            if (sq > 1.0f) continue;
+            const float half = sqrtf(1.0f - sq);
+            const float t0 = (smid - half) / norm, t1 = (smid + half) / norm;
+            float t;
+            if (t0 >= znear && t0 <= zfar)
+                t = t0;
+            else if (t1 >= znear && t1 <= zfar)
+                t = t1;
+            else
+                continue;
+            if (t < best || (t == best && g < bid)) {
+                const float tn = t * norm;
+                best = t;
+                bid = g;
+                for (int i = 0; i < 3; ++i) bn[i] = lo[i] + tn * u[i];
+            }
+        }
+        float out[3] = {bg[0], bg[1], bg[2]}, depth = 0.0f;
+        if (bid >= 0) {
+            const float *G = geo + 16 * (size_t)bid;
+            const float ln = sqrtf((bn[0] * bn[0] + bn[1] * bn[1]) + bn[2] * bn[2]);
+            const float l[3] = {bn[0] / ln, bn[1] / ln, bn[2] / ln};
+            float w[3];
+            for (int j = 0; j < 3; ++j) w[j] = (G[j] * l[0] + G[3 + j] * l[1]) + G[6 + j] * l[2];
+            const float wn = sqrtf((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
+            const float dn = sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+            float al = ((-d[0] / dn) * (w[0] / wn) + (-d[1] / dn) * (w[1] / wn)) + (-d[2] / dn) * (w[2] / wn);
+            al = al > 0.5f ? al : 0.5f;
+            for (int j = 0; j < 3; ++j) out[j] = al * (rgb[3 * (size_t)bid + j] + 0.15f);
+            depth = best;
+        }
+        for (int j = 0; j < 3; ++j) C[3 * (size_t)r + j] = out[j];
+        D[r] = depth;
+    }
+}
+
+/*
  * Forward for R explicit rays.  Outputs per ray: C[3], D, T, F, est[3],
  * ncomp and the composited list (ids, T before the hit) up to cap entries.
  * rgb is the per-Gaussian colour [n,3].  mode: 0 skip-set, 1 literal.

@@ -41,6 +41,17 @@ def make_knobs(exp_power=2.0, alpha_threshold=0.01, t_threshold=0.01, eps_fwd=1e
     return k
 
 
+def ellipsoids(geo, rgb, ro, rd, min_opacity=0.0, znear=0.0, zfar=99999.9, bg=(0.0, 0.0, 0.0)):
+    """ellipsoid_viewer.cu:1-94 (display mode): colours [R,3] and depth [R]."""
+    geo, rgb, ro, rd = _f(geo), _f(rgb), _f(ro), _f(rd)
+    r = ro.shape[0]
+    out = np.zeros((r, 3), np.float32)
+    depth = np.zeros(r, np.float32)
+    lib().or_ellipsoids(C.c_int(geo.shape[0]), _p(geo), _p(rgb), C.c_int(r), _p(ro), _p(rd),
+                        C.c_float(min_opacity), C.c_float(znear), C.c_float(zfar), _p(_f(bg)), _p(out), _p(depth))
+    return out, depth
+
+
 def fp16_round(x) -> np.ndarray:
     """float -> binary16 (round to nearest even) -> float, as the reference's
     __float2half_rn storage of hit distances and alphas

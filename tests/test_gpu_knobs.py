@@ -108,3 +108,44 @@ def test_fp16_hits_forward_backward(lists, hit_cap):
     for name, key in (("grad_opacity", "opacity"), ("grad_scale", "scale"), ("grad_rotation", "rotation"),
                       ("grad_channels", "rgb")):
         assert rel_err(getattr(g, name).cpu(), ob[key]) < 1e-3, name
+
+
+@pytest.mark.parametrize("use_bvh", [True, False])
+def test_ellipsoid_mode_matches_oracle(use_bvh):
+    """config.render_ellipsoids (reference ellipsoid_viewer.cu) through
+    forward_pass and forward_pass_display without gradients."""
+    scene = scenes.tiny_scene(1024, seed=41)
+    cam = scenes.tiny_camera(48, 40)
+    bg = (0.1, 0.2, 0.3)
+    nr = setup_native(scene, cam, bg=bg, znear=1.0, zfar=4.0)
+    nr.use_bvh = use_bvh
+    torch.set_grad_enabled(True)
+    nr.forward_pass()  # writes the rays used below
+    normal = nr.get_framebuffer().output_channels[:40, :48].clone()
+    nr.backward_pass()
+    ro, rd = rays_of(nr)
+    cfg = nr.get_config()
+    cfg.render_ellipsoids.fill_(True)
+    cfg.ellipsoid_min_opacity.fill_(0.3)
+    cfg.render_depth.fill_(True)
+    geo, _ = nr.debug_snapshot()
+    rgb = nr.get_gaussians().channels.cpu().numpy()
+    C, D = O.ellipsoids(geo.cpu().numpy(), rgb, ro, rd, 0.3, 1.0, 4.0, bg)
+    assert (D > 0).sum() > 200 and (D == 0).sum() > 50
+    for display in (False, True):
+        fb = nr.get_framebuffer()
+        fb.output_channels.zero_()
+        fb.output_depth.zero_()
+        with torch.no_grad():
+            (nr.forward_pass_display if display else nr.forward_pass)()
+        torch.cuda.synchronize()
+        nr.device_status()
+        img = fb.output_channels[:40, :48].reshape(-1, 3).cpu().numpy()
+        depth = fb.output_depth[:40, :48].reshape(-1).cpu().numpy()
+        assert np.abs(img - C).max() < 1e-5
+        assert np.abs(depth - D).max() < 1e-5 * 4.0
+    cfg.render_ellipsoids.fill_(False)
+    with torch.no_grad():
+        nr.forward_pass()
+    torch.cuda.synchronize()
+    assert torch.allclose(nr.get_framebuffer().output_channels[:40, :48], normal, atol=1e-6)

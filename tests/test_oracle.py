@@ -436,3 +436,40 @@ def test_fp16_hits_literal_and_skip_set_agree(oracle_lib):
     assert np.abs(a["F"] - b["F"]).max() < 1e-5
     f32 = O.render(geo, colors, ro, rd, O.make_knobs(t_threshold=0.2, bg=(0.3, 0.1, 0.2)), mode=0)
     assert np.abs(a["C"] - f32["C"]).max() > 1e-5
+
+
+# ---------------------------------------------------------------------------
+# ellipsoid display mode (ellipsoid_viewer.cu)
+# ---------------------------------------------------------------------------
+def test_ellipsoid_mode_sphere_closed_form(oracle_lib):
+    # a sphere of radius rho = exp(s) * clip at the origin, rays along -z
+    # from (b, 0, 5): entry t = 5 - sqrt(rho^2 - b^2), world normal
+    # (b, 0, sqrt(rho^2 - b^2)) / rho, shade max(0.5, cos) (c + 0.15)
+    s, logit = -0.5, 1.0
+    op = 1.0 / (1.0 + np.exp(-logit))
+    clip = (4.0 * np.log(op / 0.01)) ** 0.25
+    rho = np.exp(s) * clip
+    geo, _ = O.snapshot(np.zeros((1, 3)), np.array([[1.0, 0.0, 0.0, 0.0]]), np.full((1, 3), s),
+                        np.array([[logit]]), 0.01, 2.0)
+    rgb = np.array([[0.2, 0.5, 0.7]], np.float32)
+    bs = np.array([0.0, 0.3 * rho, 0.9 * rho, 1.1 * rho])
+    ro = np.stack([bs, np.zeros(4), np.full(4, 5.0)], 1).astype(np.float32)
+    rd = np.tile(np.array([[0.0, 0.0, -1.0]], np.float32), (4, 1))
+    bg = (0.05, 0.1, 0.15)
+    C, D = O.ellipsoids(geo, rgb, ro, rd, 0.0, 0.0, 99999.9, bg)
+    h = np.sqrt(np.maximum(rho ** 2 - bs ** 2, 0.0))
+    for i in range(3):
+        assert abs(D[i] - (5.0 - h[i])) < 2e-6 * 5
+        assert np.abs(C[i] - max(0.5, h[i] / rho) * (rgb[0] + 0.15)).max() < 2e-6
+    assert D[3] == 0.0 and np.array_equal(C[3], np.array(bg, np.float32))
+    # znear past the entry point: the exit point, facing away (shade 0.5)
+    C, D = O.ellipsoids(geo, rgb, ro[:1], rd[:1], 0.0, 5.0, 99999.9, bg)
+    assert abs(D[0] - (5.0 + rho)) < 1e-5 and np.abs(C[0] - 0.5 * (rgb[0] + 0.15)).max() < 2e-6
+    # below the opacity threshold: background
+    C, D = O.ellipsoids(geo, rgb, ro[:1], rd[:1], op + 0.01, 0.0, 99999.9, bg)
+    assert D[0] == 0.0 and np.array_equal(C[0], np.array(bg, np.float32))
+    # two spheres on the ray: the nearer one
+    geo2, _ = O.snapshot(np.array([[0.0, 0.0, -2.0], [0.0, 0.0, 0.0]]), np.array([[1.0, 0, 0, 0]] * 2),
+                         np.full((2, 3), s), np.array([[logit], [logit]]), 0.01, 2.0)
+    C, D = O.ellipsoids(geo2, np.array([[1.0, 0, 0], [0, 1.0, 0]], np.float32), ro[:1], rd[:1])
+    assert abs(D[0] - (5.0 - rho)) < 1e-5 and C[0, 1] > C[0, 0]
