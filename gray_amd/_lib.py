@@ -33,7 +33,7 @@ class Knobs(C.Structure):
     _fields_ = [
         ("exp_power", vp), ("alpha_threshold", vp), ("t_threshold", vp), ("global_scale_factor", vp),
         ("eps_forward_normalization", vp), ("eps_scale_grad", vp), ("background_channels", vp),
-        ("render_depth", vp), ("enable_sh", vp),
+        ("render_depth", vp), ("enable_sh", vp), ("jitter_primary_rays", vp), ("call_index", vp),
     ]
 
 
@@ -132,7 +132,7 @@ def _load():
     return lib
 
 
-ABI_VERSION = 5  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
+ABI_VERSION = 6  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
 LIB = _load()
 if LIB.gray_abi_version() != ABI_VERSION:
     raise ImportError(f"libgray_hip.so has ABI {LIB.gray_abi_version()}, the binding expects {ABI_VERSION}; "

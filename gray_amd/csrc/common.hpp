@@ -49,7 +49,29 @@ struct __attribute__((aligned(16))) BvhNode {
 struct DevKnobs {
     const float *exp_power, *alpha_threshold, *t_threshold, *gsf, *eps_fwd, *eps_scale_grad, *bg;
     const uint8_t *render_depth, *enable_sh;
+    const uint8_t *jitter;  // may be NULL
+    const int32_t *calls;   // may be NULL
 };
+
+// Sub-pixel jitter stream of the reference camera (random.h, camera.h:17-36):
+// the seed is the Tiny Encryption Algorithm (Wheeler and Needham, 1994) with
+// 4 rounds on (pixel, call index) under the key of its published test, the
+// offsets are draws of the Numerical Recipes LCG (a = 1664525,
+// c = 1013904223), 24 bits / 2^24.
+__host__ __device__ __forceinline__ uint32_t gray_tea4(uint32_t v0, uint32_t v1) {
+    uint32_t sum = 0;
+    for (int n = 0; n < 4; ++n) {
+        sum += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + sum) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + sum) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+
+__host__ __device__ __forceinline__ float gray_lcg_unit(uint32_t &state) {
+    state = 1664525u * state + 1013904223u;
+    return (float)(state & 0x00FFFFFFu) / (float)0x01000000;
+}
 
 struct DevView {
     const float *origin, *c2w, *fov, *znear, *zfar, *rays_o, *rays_d, *rgb;

@@ -76,7 +76,7 @@ __device__ __forceinline__ bool block_tile(const TraceArgs &A, int &view, int &t
 // Primary ray of pixel (x, y): reference Camera::compute_primary_ray_direction
 // (camera.h:17-36) without jitter; explicit rays when the view provides them
 // (shaders.cu:80-85).  Returns false for an inactive pixel (zero direction).
-__device__ __forceinline__ bool make_ray(const DevView &V, int x, int y, LaneRay &r) {
+__device__ __forceinline__ bool make_ray(const DevKnobs &kn, const DevView &V, int x, int y, LaneRay &r) {
 #pragma clang fp contract(off)
     const int pix = y * V.stride + x;
     float dx, dy, dz;
@@ -93,8 +93,14 @@ __device__ __forceinline__ bool make_ray(const DevView &V, int x, int y, LaneRay
         r.oz = V.origin[2];
         const float vs = tanf(*V.fov / 2.0f);
         const float aspect = (float)V.width / (float)V.height;
-        const float yf = vs * (1.0f - (2.0f * ((float)y + 0.5f)) / (float)V.height);
-        const float xf = (aspect * vs) * ((2.0f * ((float)x + 0.5f)) / (float)V.width - 1.0f);
+        float px = (float)x, py = (float)y;
+        if (kn.jitter && *kn.jitter) {
+            uint32_t seed = gray_tea4((uint32_t)pix, kn.calls ? (uint32_t)*kn.calls : 0u);
+            px += gray_lcg_unit(seed) - 0.5f;
+            py += gray_lcg_unit(seed) - 0.5f;
+        }
+        const float yf = vs * (1.0f - (2.0f * (py + 0.5f)) / (float)V.height);
+        const float xf = (aspect * vs) * ((2.0f * (px + 0.5f)) / (float)V.width - 1.0f);
         const float *c = V.c2w;  // row-major; direction = col0*x + col1*y - col2
         dx = (c[0] * xf + c[1] * yf) - c[2];
         dy = (c[3] * xf + c[4] * yf) - c[5];
@@ -557,7 +563,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
     const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
     LaneRay r{};
     bool active = false;
-    if (x < V.width && y < V.height) active = make_ray(V, x, y, r);
+    if (x < V.width && y < V.height) active = make_ray(A.knobs, V, x, y, r);
     const size_t g = (size_t)tile * GRAY_WAVE + lane;
     int cnt = 0, visits = 0, leaves = 0;
     bool ok = true;
@@ -621,7 +627,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const 
                 g = task.x;
                 int x, y;
                 const DevView &V = lane_pixel(A, g, x, y);
-                if (x < V.width && y < V.height && A.list_n[g] >= 0 && make_ray(V, x, y, r)) start = task.y;
+                if (x < V.width && y < V.height && A.list_n[g] >= 0 && make_ray(A.knobs, V, x, y, r)) start = task.y;
             }
         }
         int cnt = 0, visits = 0, leaves = 0;
@@ -715,7 +721,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
     const bool inside = x < V.width && y < V.height;
     LaneRay r{};
     bool active = false;
-    if (inside) active = make_ray(V, x, y, r);
+    if (inside) active = make_ray(A.knobs, V, x, y, r);
     const bool lists = A.list_cap > 0;
     const size_t lg = (size_t)tile * GRAY_WAVE + lane;
     const int lcount = lists ? A.list_n[lg] : -1;
@@ -948,7 +954,7 @@ __global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) 
     const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
     LaneRay r{};
     bool active = false;
-    if (x < V.width && y < V.height) active = make_ray(V, x, y, r);
+    if (x < V.width && y < V.height) active = make_ray(A.knobs, V, x, y, r);
     const size_t lg = (size_t)tile * GRAY_WAVE + lane;
     const int n = A.list_n[lg];
     active = active && n >= 0;
@@ -1497,7 +1503,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_debug_hits(TraceArgs A, const Geo
     const bool inside = x < V.width && y < V.height;
     LaneRay r{};
     bool active = false;
-    if (inside) active = make_ray(V, x, y, r);
+    if (inside) active = make_ray(A.knobs, V, x, y, r);
     const float athr = *A.knobs.alpha_threshold;
     const int ray = y * V.width + x;
     int cnt = 0;
@@ -1534,7 +1540,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_ellipsoids(TraceArgs A, const Geo
     const bool inside = x < V.width && y < V.height;
     LaneRay r{};
     bool active = false;
-    if (inside) active = make_ray(V, x, y, r);
+    if (inside) active = make_ray(A.knobs, V, x, y, r);
     const float znear = *V.znear, zfar = *V.zfar, mo = *min_op;
     float best = INFINITY, bnx = 0.0f, bny = 0.0f, bnz = 0.0f;
     uint32_t bid = GRAY_EMPTY;
@@ -1611,7 +1617,7 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
     }
     A.knobs = DevKnobs{k.exp_power, k.alpha_threshold, k.t_threshold, k.global_scale_factor,
                        k.eps_forward_normalization, k.eps_scale_grad, k.background_channels, k.render_depth,
-                       k.enable_sh};
+                       k.enable_sh, k.jitter_primary_rays, k.call_index};
     int tiles = 0;
     for (int i = 0; i < nviews; ++i) {
         const gray_view &v = args->views[first_view + i];

@@ -77,12 +77,14 @@ class ConfigDataHolder:
         self.needs_ray_output = b(False)
         self.update_channels = b(True)
 
-    def knobs(self) -> L.Knobs:
+    def knobs(self, call_index=None) -> L.Knobs:
+        """call_index: the metadata's total_num_calls (seed of the ray jitter)."""
         return L.Knobs(
             self.exp_power.data_ptr(), self.alpha_threshold.data_ptr(), self.t_threshold.data_ptr(),
             self.global_scale_factor.data_ptr(), self.eps_forward_normalization.data_ptr(),
             self.eps_scale_grad.data_ptr(), self.background_channels.data_ptr(), self.render_depth.data_ptr(),
-            self.enable_sh.data_ptr(),
+            self.enable_sh.data_ptr(), self.jitter_primary_rays.data_ptr(),
+            call_index.data_ptr() if call_index is not None else None,
         )
 
 
@@ -371,7 +373,7 @@ class NativeRaytracer:
 
     def _args(self, view: L.View, save: bool) -> L.RenderArgs:
         a = L.RenderArgs()
-        a.knobs = self.config_data.knobs()
+        a.knobs = self.config_data.knobs(self.meta_data.total_num_calls)
         a.num_views = 1
         self._view_keep = (view,)
         a.views = C.pointer(view)

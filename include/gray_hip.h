@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 5
+#define GRAY_ABI_VERSION 6
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -53,6 +53,9 @@ typedef struct {
     const float *background_channels;       /* [3] */
     const uint8_t *render_depth;            /* [1] bool */
     const uint8_t *enable_sh;               /* [1] bool */
+    const uint8_t *jitter_primary_rays;     /* [1] bool or NULL: sub-pixel jitter of camera rays (camera.h:17-36) */
+    const int32_t *call_index;              /* [1] or NULL (0): jitter seed, the reference's metadata
+                                               total_num_calls (shaders.cu:76) */
 } gray_knobs;
 
 /* Gaussian parameters (read by every stage) */

@@ -139,6 +139,53 @@ void or_snapshot(int n, const float *mean, const float *rot, const float *scale,
     }
 }
 
+/* Tiny Encryption Algorithm (Wheeler and Needham, 1994): `rounds` cycles on
+ * (v0, v1) under key k; returns the first word.  The reference seeds its ray
+ * jitter with 4 rounds under the key of the published test (random.h). */
+uint32_t or_tea(uint32_t v0, uint32_t v1, int rounds, const uint32_t *k) {
+    uint32_t sum = 0;
+    for (int n = 0; n < rounds; ++n) {
+        sum += 0x9e3779b9u;
+        v0 += ((v1 << 4) + k[0]) ^ (v1 + sum) ^ ((v1 >> 5) + k[1]);
+        v1 += ((v0 << 4) + k[2]) ^ (v0 + sum) ^ ((v0 >> 5) + k[3]);
+    }
+    return v0;
+}
+
+/* Numerical Recipes LCG step (a = 1664525, c = 1013904223, mod 2^32) */
+uint32_t or_lcg(uint32_t state) { return 1664525u * state + 1013904223u; }
+
+/* camera.h:17-36 with the sub-pixel jitter: offsets (draw - 0.5) for x then y,
+ * draws (lcg & 0xFFFFFF) / 2^24 from the seed tea4(pixel, call index) */
+void or_pinhole_rays_jittered(int W, int H, int stride, float fov, const float *origin, const float *c,
+                              uint32_t calls, float *ro, float *rd) {
+    static const uint32_t key[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
+    const float vs = tanf(fov / 2.0f);
+    const float aspect = (float)W / (float)H;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            uint32_t st = or_tea((uint32_t)(y * stride + x), calls, 4, key);
+            st = or_lcg(st);
+            const float jx = (float)(st & 0x00FFFFFFu) / (float)0x01000000 - 0.5f;
+            st = or_lcg(st);
+            const float jy = (float)(st & 0x00FFFFFFu) / (float)0x01000000 - 0.5f;
+            const float px = (float)x + jx, py = (float)y + jy;
+            const float yf = vs * (1.0f - (2.0f * (py + 0.5f)) / (float)H);
+            const float xf = (aspect * vs) * ((2.0f * (px + 0.5f)) / (float)W - 1.0f);
+            float d0 = (c[0] * xf + c[1] * yf) - c[2];
+            float d1 = (c[3] * xf + c[4] * yf) - c[5];
+            float d2 = (c[6] * xf + c[7] * yf) - c[8];
+            const float len = sqrtf((d0 * d0 + d1 * d1) + d2 * d2);
+            const size_t i = (size_t)y * W + x;
+            ro[3 * i] = origin[0];
+            ro[3 * i + 1] = origin[1];
+            ro[3 * i + 2] = origin[2];
+            rd[3 * i] = d0 / len;
+            rd[3 * i + 1] = d1 / len;
+            rd[3 * i + 2] = d2 / len;
+        }
+}
+
 void or_pinhole_rays(int W, int H, int stride, float fov, const float *origin, const float *c, float *ro, float *rd) {
     const float vs = tanf(fov / 2.0f);
     const float aspect = (float)W / (float)H;

@@ -108,6 +108,26 @@ def pinhole_rays(width, height, fov, origin, c2w):
     return ro, rd
 
 
+def pinhole_rays_jittered(width, height, fov, origin, c2w, calls):
+    """camera.h:17-36 with the sub-pixel jitter seeded by (pixel, calls)."""
+    ro = np.zeros((height * width, 3), np.float32)
+    rd = np.zeros((height * width, 3), np.float32)
+    lib().or_pinhole_rays_jittered(C.c_int(width), C.c_int(height), C.c_int(width), C.c_float(fov),
+                                   _p(_f(origin)), _p(_f(c2w)), C.c_uint32(calls), _p(ro), _p(rd))
+    return ro, rd
+
+
+def tea(v0, v1, rounds, key) -> int:
+    lib().or_tea.restype = C.c_uint32
+    k = (C.c_uint32 * 4)(*key)
+    return int(lib().or_tea(C.c_uint32(v0), C.c_uint32(v1), C.c_int(rounds), k))
+
+
+def lcg(state) -> int:
+    lib().or_lcg.restype = C.c_uint32
+    return int(lib().or_lcg(C.c_uint32(state)))
+
+
 def hitlists(geo, ro, rd, alpha_threshold=0.01, max_hits=None):
     """All hits of each ray sorted by (distance, id) (shaders.cu:22-47)."""
     geo, ro, rd = _f(geo), _f(ro), _f(rd)

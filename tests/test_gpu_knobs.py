@@ -149,3 +149,31 @@ def test_ellipsoid_mode_matches_oracle(use_bvh):
         nr.forward_pass()
     torch.cuda.synchronize()
     assert torch.allclose(nr.get_framebuffer().output_channels[:40, :48], normal, atol=1e-6)
+
+
+def test_jittered_primary_rays_match_oracle():
+    """config.jitter_primary_rays: the rays follow the reference jitter
+    stream seeded by (pixel, metadata total_num_calls)."""
+    scene = scenes.tiny_scene(1024, seed=51)
+    cam = scenes.tiny_camera(48, 40)
+    nr = setup_native(scene, cam)
+    nr.get_config().jitter_primary_rays.fill_(True)
+    torch.set_grad_enabled(True)
+    dirs = []
+    for _ in range(2):
+        nr.forward_pass()
+        torch.cuda.synchronize()
+        calls = int(nr.get_metadata().total_num_calls.item())
+        ro, rd = rays_of(nr)
+        oro, ord_ = O.pinhole_rays_jittered(48, 40, cam.fov_y, cam.origin.astype(np.float32),
+                                            cam.rotation_c2w_blender(), calls)
+        assert np.abs(ro - oro).max() == 0.0
+        assert np.abs(rd - ord_).max() < 2e-6
+        dirs.append(rd)
+        geo, _ = nr.debug_snapshot()
+        ref = O.render(geo.cpu().numpy(), nr.get_gaussians().channels.cpu().numpy(), ro, rd,
+                       O.make_knobs(t_threshold=0.03))
+        img = nr.get_framebuffer().output_channels[:40, :48].reshape(-1, 3).cpu().numpy()
+        assert np.abs(img - ref["C"]).max() < 1e-5
+        nr.backward_pass()
+    assert np.abs(dirs[0] - dirs[1]).max() > 1e-4, "a new call draws new offsets"

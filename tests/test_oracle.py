@@ -473,3 +473,39 @@ def test_ellipsoid_mode_sphere_closed_form(oracle_lib):
                          np.full((2, 3), s), np.array([[logit], [logit]]), 0.01, 2.0)
     C, D = O.ellipsoids(geo2, np.array([[1.0, 0, 0], [0, 1.0, 0]], np.float32), ro[:1], rd[:1])
     assert abs(D[0] - (5.0 - rho)) < 1e-5 and C[0, 1] > C[0, 0]
+
+
+# ---------------------------------------------------------------------------
+# sub-pixel ray jitter (camera.h:17-36, random.h)
+# ---------------------------------------------------------------------------
+def test_jitter_generators_known_answers(oracle_lib):
+    # TEA's published test vector: zero key, zero plaintext, 32 cycles -> 0x41ea3a0a (first word)
+    assert O.tea(0, 0, 32, [0, 0, 0, 0]) == 0x41EA3A0A
+    # Numerical Recipes LCG from 0
+    seq, x = [], 0
+    for _ in range(3):
+        x = O.lcg(x)
+        seq.append(x)
+    assert seq == [1013904223, 1196435762, 3519870697]
+
+
+def test_jittered_rays_stay_within_the_pixel(oracle_lib):
+    W, H, fov = 40, 30, 1.0
+    origin = np.array([0.0, 0.0, 3.0], np.float32)
+    c2w = np.diag([1.0, 1.0, 1.0]).astype(np.float32)
+    ro, rd = O.pinhole_rays(W, H, fov, origin, c2w)
+    ro1, rd1 = O.pinhole_rays_jittered(W, H, fov, origin, c2w, 7)
+    _, rd2 = O.pinhole_rays_jittered(W, H, fov, origin, c2w, 8)
+    assert np.array_equal(ro, ro1)
+    assert not np.allclose(rd1, rd2)
+    # back to pixel coordinates: the offsets lie in [-0.5, 0.5) and average 0
+    vs = np.tan(fov / 2)
+    def pix(d):
+        x = d[:, 0] / -d[:, 2]
+        y = d[:, 1] / -d[:, 2]
+        return (x / (vs * W / H) + 1) * W / 2 - 0.5, (1 - y / vs) * H / 2 - 0.5
+    px0, py0 = pix(rd)
+    px1, py1 = pix(rd1)
+    jx, jy = px1 - px0, py1 - py0
+    assert np.all(np.abs(jx) <= 0.5 + 1e-4) and np.all(np.abs(jy) <= 0.5 + 1e-4)
+    assert abs(jx.mean()) < 0.05 and abs(jy.mean()) < 0.05 and jx.std() > 0.2
