@@ -34,6 +34,7 @@ class Knobs(C.Structure):
         ("exp_power", vp), ("alpha_threshold", vp), ("t_threshold", vp), ("global_scale_factor", vp),
         ("eps_forward_normalization", vp), ("eps_scale_grad", vp), ("background_channels", vp),
         ("render_depth", vp), ("enable_sh", vp), ("jitter_primary_rays", vp), ("call_index", vp),
+        ("needs_ray_output", vp),
     ]
 
 

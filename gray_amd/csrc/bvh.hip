@@ -770,7 +770,8 @@ extern "C" int gray_bvh_update(gray_ctx *ctx, const gray_params *params, const g
     if (timed) GRAY_HIP(hipEventRecord(ctx->ev0, s));
     DevKnobs kn{knobs->exp_power, knobs->alpha_threshold, knobs->t_threshold, knobs->global_scale_factor,
                 knobs->eps_forward_normalization, knobs->eps_scale_grad, knobs->background_channels,
-                knobs->render_depth, knobs->enable_sh, knobs->jitter_primary_rays, knobs->call_index};
+                knobs->render_depth, knobs->enable_sh, knobs->jitter_primary_rays, knobs->call_index,
+                knobs->needs_ray_output};
     hipLaunchKernelGGL(k_snapshot, dim3(gray_div_up(n, 256)), dim3(256), 0, s, (int)n, params->mean,
                        params->rotation, params->scale, params->opacity, kn, ctx->geo, ctx->l2w, ctx->aabb);
     GRAY_HIP(hipGetLastError());

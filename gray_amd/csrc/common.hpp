@@ -51,6 +51,7 @@ struct DevKnobs {
     const uint8_t *render_depth, *enable_sh;
     const uint8_t *jitter;  // may be NULL
     const int32_t *calls;   // may be NULL
+    const uint8_t *needs_ray;  // may be NULL
 };
 
 // Sub-pixel jitter stream of the reference camera (random.h, camera.h:17-36):

@@ -671,7 +671,16 @@ __device__ __forceinline__ void store_pixel(const TraceArgs &A, const DevView &V
     V.out_c[3 * pix + 0] = C0;
     V.out_c[3 * pix + 1] = C1;
     V.out_c[3 * pix + 2] = C2;
-    if (V.out_depth) V.out_depth[pix] = D;
+    // reference Framebuffer::write (framebuffer.h:33-52): everything when the
+    // forward saves state; otherwise depth if asked for or if the ray output
+    // is, and the ray direction with the ray output
+    const bool ray_out = !A.save && A.knobs.needs_ray && *A.knobs.needs_ray;
+    if (V.out_depth && (A.save || depth_on || ray_out)) V.out_depth[pix] = D;
+    if (ray_out && V.out_rd) {
+        V.out_rd[3 * pix + 0] = r.dx;
+        V.out_rd[3 * pix + 1] = r.dy;
+        V.out_rd[3 * pix + 2] = r.dz;
+    }
     if (A.save) {
         if (V.out_T) V.out_T[pix] = T;
         if (V.out_F) V.out_F[pix] = F;
@@ -1617,7 +1626,7 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
     }
     A.knobs = DevKnobs{k.exp_power, k.alpha_threshold, k.t_threshold, k.global_scale_factor,
                        k.eps_forward_normalization, k.eps_scale_grad, k.background_channels, k.render_depth,
-                       k.enable_sh, k.jitter_primary_rays, k.call_index};
+                       k.enable_sh, k.jitter_primary_rays, k.call_index, k.needs_ray_output};
     int tiles = 0;
     for (int i = 0; i < nviews; ++i) {
         const gray_view &v = args->views[first_view + i];

@@ -84,7 +84,7 @@ class ConfigDataHolder:
             self.global_scale_factor.data_ptr(), self.eps_forward_normalization.data_ptr(),
             self.eps_scale_grad.data_ptr(), self.background_channels.data_ptr(), self.render_depth.data_ptr(),
             self.enable_sh.data_ptr(), self.jitter_primary_rays.data_ptr(),
-            call_index.data_ptr() if call_index is not None else None,
+            call_index.data_ptr() if call_index is not None else None, self.needs_ray_output.data_ptr(),
         )
 
 
@@ -355,6 +355,7 @@ class NativeRaytracer:
         v.rgb = g.channels.data_ptr()
         v.out_channels = fb.output_channels.data_ptr()
         v.out_depth = fb.output_depth.data_ptr()
+        v.out_ray_direction = fb.ray_direction.data_ptr()  # without saved state: only with needs_ray_output
         if save:
             v.out_transmittance = fb.transmittance.data_ptr()
             v.out_full_transmittance = fb.full_transmittance.data_ptr()

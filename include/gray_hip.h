@@ -56,6 +56,8 @@ typedef struct {
     const uint8_t *jitter_primary_rays;     /* [1] bool or NULL: sub-pixel jitter of camera rays (camera.h:17-36) */
     const int32_t *call_index;              /* [1] or NULL (0): jitter seed, the reference's metadata
                                                total_num_calls (shaders.cu:76) */
+    const uint8_t *needs_ray_output;        /* [1] bool or NULL: without saved state, still write depth and
+                                               the ray direction (framebuffer.h:46-51) */
 } gray_knobs;
 
 /* Gaussian parameters (read by every stage) */

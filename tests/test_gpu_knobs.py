@@ -177,3 +177,34 @@ def test_jittered_primary_rays_match_oracle():
         assert np.abs(img - ref["C"]).max() < 1e-5
         nr.backward_pass()
     assert np.abs(dirs[0] - dirs[1]).max() > 1e-4, "a new call draws new offsets"
+
+
+def test_framebuffer_write_rules_without_gradients():
+    """Reference Framebuffer::write (framebuffer.h:33-52): without gradients
+    only the colour, depth when render_depth or needs_ray_output is set, and
+    the ray direction when needs_ray_output is set."""
+    scene = scenes.tiny_scene(1024, seed=61)
+    cam = scenes.tiny_camera(48, 40)
+    nr = setup_native(scene, cam)
+    torch.set_grad_enabled(True)
+    nr.forward_pass()
+    fb = nr.get_framebuffer()
+    rd = fb.ray_direction.clone()
+    nr.backward_pass()
+    cfg = nr.get_config()
+    for render_depth, ray_out in ((False, False), (True, False), (False, True)):
+        cfg.render_depth.fill_(render_depth)
+        cfg.needs_ray_output.fill_(ray_out)
+        fb.output_depth.fill_(-7.0)
+        fb.ray_direction.fill_(-7.0)
+        fb.transmittance.fill_(-7.0)
+        with torch.no_grad():
+            nr.forward_pass()
+        torch.cuda.synchronize()
+        depth_written = bool((fb.output_depth[:40, :48] != -7.0).all())
+        assert depth_written == (render_depth or ray_out)
+        if ray_out:
+            assert torch.equal(fb.ray_direction[:40, :48], rd[:40, :48])
+        else:
+            assert bool((fb.ray_direction == -7.0).all())
+        assert bool((fb.transmittance == -7.0).all())
