@@ -334,7 +334,7 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 #endif
 // Nodes visited and leaves tested per lane and iteration.
 #ifndef GRAY_COLLECT_HEADS
-#define GRAY_COLLECT_HEADS 2
+#define GRAY_COLLECT_HEADS 1
 #endif
 #ifndef GRAY_COLLECT_LEAVES
 #define GRAY_COLLECT_LEAVES 1
@@ -347,10 +347,10 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 // Iterations a walk runs before it hands its remaining work on as tasks, in
 // the per-tile pass and in every task round but the last (which finishes).
 #ifndef GRAY_COLLECT_BUDGET
-#define GRAY_COLLECT_BUDGET 192
+#define GRAY_COLLECT_BUDGET 384
 #endif
 #ifndef GRAY_TASK_BUDGET
-#define GRAY_TASK_BUDGET 96
+#define GRAY_TASK_BUDGET 192
 #endif
 #ifndef GRAY_TASK_ROUNDS
 #define GRAY_TASK_ROUNDS 3
