@@ -118,6 +118,7 @@ def _load():
         "gray_debug_hitlists": (i32, [vp, C.POINTER(RenderArgs), i32, vp, vp, vp, vp]),
         "gray_debug_snapshot": (i32, [vp, vp, vp, vp]),
         "gray_debug_bvh": (i32, [vp, vp, vp, vp, vp]),
+        "gray_debug_wide": (i32, [vp, vp, vp, vp]),
         "gray_debug_list_counts": (i32, [vp, vp, i64, C.POINTER(i64), vp]),
         "gray_render_stage_ms": (i32, [vp, C.POINTER(C.c_float), i32]),
         "gray_render_reserve": (i32, [vp, i64, i32]),
@@ -133,7 +134,7 @@ def _load():
     return lib
 
 
-ABI_VERSION = 6  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
+ABI_VERSION = 7  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
 LIB = _load()
 if LIB.gray_abi_version() != ABI_VERSION:
     raise ImportError(f"libgray_hip.so has ABI {LIB.gray_abi_version()}, the binding expects {ABI_VERSION}; "
@@ -144,7 +145,7 @@ EXPORTED = (
     "gray_sh_backward", "gray_render_forward", "gray_render_backward", "gray_adam_step", "gray_debug_hitlists",
     "gray_debug_snapshot", "gray_debug_bvh", "gray_debug_list_counts", "gray_render_stage_ms",
     "gray_render_reserve", "gray_loss_workspace_bytes", "gray_loss_forward", "gray_loss_backward",
-    "gray_render_ellipsoids",
+    "gray_render_ellipsoids", "gray_debug_wide",
 )
 
 

@@ -22,6 +22,8 @@
 
 #include <algorithm>
 
+#include <hip/hip_fp16.h>
+
 #include "ctx.hpp"
 
 namespace {
@@ -604,6 +606,73 @@ __global__ void k_refit_top(int count, const uint32_t *__restrict__ top_nodes, c
     for (int i = 0; i < 6; ++i) slot[i] = box[i];
 }
 
+// ---------------------------------------------------------------------------
+// 4-wide nodes of the collect pass.  Topology part, after every rebuild: an
+// internal node is a 4-wide node when its depth is even (the root's is 0), so
+// a 4-wide node's internal children are the binary nodes two levels down.  The
+// depth is counted up the parent chain; it is below 64, and the bound of the
+// loop only keeps it finite.
+// ---------------------------------------------------------------------------
+__global__ void k_wide_flags(int count, const uint4 *__restrict__ node_info, uint8_t *__restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    uint32_t cur = (uint32_t)i;
+    int d = 0;
+    for (int k = 0; k < 128 && cur != 0u && cur < (uint32_t)count; ++k) {
+        cur = node_info[cur].w & 0x7fffffffu;
+        ++d;
+    }
+    flag[i] = (uint8_t)((d & 1) == 0);
+}
+
+// Box part, after every refit: the children of 4-wide node i are, per side of
+// binary node i, the two children of an internal child (slots 2s, 2s+1) or the
+// leaf child itself (slot 2s, slot 2s+1 unused).  The boxes are the refitted
+// binary child boxes rounded outward to half precision.
+__global__ void k_widen(int count, const uint8_t *__restrict__ flag, const BvhNode *__restrict__ nodes,
+                        Bvh4Node *__restrict__ wide) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count || !flag[i]) return;
+    const BvhNode nd = nodes[i];
+    float lo[3][4], hi[3][4];
+    uint32_t ch[4];
+    auto put = [&](int k, const float *b, uint32_t ref) {
+        for (int a = 0; a < 3; ++a) {
+            lo[a][k] = b[a];
+            hi[a][k] = b[3 + a];
+        }
+        ch[k] = ref;
+    };
+    float none[6];
+    box_empty(none);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t c = side ? nd.right : nd.left;
+        if (c != GRAY_EMPTY && !(c & GRAY_LEAF)) {
+            const BvhNode cn = nodes[c];
+            put(2 * side, cn.lb, cn.left);
+            put(2 * side + 1, cn.rb, cn.right);
+        } else {
+            put(2 * side, side ? nd.rb : nd.lb, c);
+            put(2 * side + 1, none, GRAY_EMPTY);
+        }
+    }
+    // round toward -inf / +inf: a box never shrinks (+-inf stay +-inf, so an
+    // empty box stays empty)
+    auto down = [](float x, float y) {
+        return (uint32_t)__half_as_ushort(__float2half_rd(x)) | ((uint32_t)__half_as_ushort(__float2half_rd(y)) << 16);
+    };
+    auto up = [](float x, float y) {
+        return (uint32_t)__half_as_ushort(__float2half_ru(x)) | ((uint32_t)__half_as_ushort(__float2half_ru(y)) << 16);
+    };
+    uint4 *d = reinterpret_cast<uint4 *>(wide + i);
+    d[0] = make_uint4(down(lo[0][0], lo[0][1]), down(lo[0][2], lo[0][3]), down(lo[1][0], lo[1][1]),
+                      down(lo[1][2], lo[1][3]));
+    d[1] = make_uint4(down(lo[2][0], lo[2][1]), down(lo[2][2], lo[2][3]), up(hi[0][0], hi[0][1]), up(hi[0][2], hi[0][3]));
+    d[2] = make_uint4(up(hi[1][0], hi[1][1]), up(hi[1][2], hi[1][3]), up(hi[2][0], hi[2][1]), up(hi[2][2], hi[2][3]));
+    d[3] = make_uint4(ch[0], ch[1], ch[2], ch[3]);
+}
+
 template <typename T>
 int grow(T **ptr, int64_t count) {
     if (*ptr) {
@@ -653,6 +722,8 @@ int gray_ctx_reserve(gray_ctx *ctx, int64_t n) {
     G(top_nodes, cap);
     G(need_ends, cap * 2);
     G(node_info, cap);
+    G(wide, cap);
+    G(wide_flag, cap);
     const int64_t tiles = gray_div_up(cap, kSortTile);
     const int64_t hm = 256 * tiles;
     G(hist, hm);
@@ -684,6 +755,10 @@ int gray_refit(gray_ctx *ctx, hipStream_t s) {
                            ctx->sparse, chunks, ctx->nodes);
         GRAY_HIP(hipGetLastError());
     }
+    const int internal = n > 1 ? n - 1 : 1;
+    hipLaunchKernelGGL(k_widen, dim3(gray_div_up(internal, 256)), dim3(256), 0, s, internal, ctx->wide_flag,
+                       ctx->nodes, ctx->wide);
+    GRAY_HIP(hipGetLastError());
     return GRAY_OK;
 }
 
@@ -693,6 +768,7 @@ static int build_topology(gray_ctx *ctx, const float *mean, hipStream_t s) {
     if (n == 1) {
         hipLaunchKernelGGL(k_single_leaf, dim3(1), dim3(1), 0, s, ctx->nodes, ctx->parent_int, ctx->parent_leaf,
                            ctx->range_lo, ctx->range_hi, ctx->sorted_ids, ctx->keys);
+        hipLaunchKernelGGL(k_wide_flags, dim3(1), dim3(64), 0, s, 1, ctx->node_info, ctx->wide_flag);
         GRAY_HIP(hipGetLastError());
         ctx->num_top = 0;
         return GRAY_OK;
@@ -726,6 +802,8 @@ static int build_topology(gray_ctx *ctx, const float *mean, hipStream_t s) {
     hipLaunchKernelGGL(k_karras, dim3(gray_div_up(n - 1, 256)), dim3(256), 0, s, n, ctx->keys, ctx->sorted_ids,
                        ctx->nodes, ctx->parent_int, ctx->parent_leaf, ctx->range_lo, ctx->range_hi, ctx->top_nodes,
                        ctx->top_count, ctx->need_ends, ctx->node_info);
+    hipLaunchKernelGGL(k_wide_flags, dim3(gray_div_up(n - 1, 256)), dim3(256), 0, s, n - 1, ctx->node_info,
+                       ctx->wide_flag);
     GRAY_HIP(hipGetLastError());
     uint32_t top = 0;
     GRAY_HIP(hipMemcpyAsync(&top, ctx->top_count, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -816,6 +894,17 @@ extern "C" int gray_debug_bvh(gray_ctx *ctx, float *nodes, uint32_t *sorted_ids,
     if (nodes) GRAY_HIP(hipMemcpyAsync(nodes, ctx->nodes, sizeof(BvhNode) * nn, hipMemcpyDeviceToDevice, s));
     if (sorted_ids) GRAY_HIP(hipMemcpyAsync(sorted_ids, ctx->sorted_ids, 4 * n, hipMemcpyDeviceToDevice, s));
     if (keys) GRAY_HIP(hipMemcpyAsync(keys, ctx->keys, 4 * n, hipMemcpyDeviceToDevice, s));
+    return GRAY_OK;
+}
+
+extern "C" int gray_debug_wide(gray_ctx *ctx, uint32_t *nodes, uint8_t *flags, void *stream) {
+    if (!ctx) return GRAY_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t n = ctx->n;
+    if (n == 0) return GRAY_OK;
+    const int64_t nn = n > 1 ? n - 1 : 1;
+    if (nodes) GRAY_HIP(hipMemcpyAsync(nodes, ctx->wide, sizeof(Bvh4Node) * nn, hipMemcpyDeviceToDevice, s));
+    if (flags) GRAY_HIP(hipMemcpyAsync(flags, ctx->wide_flag, nn, hipMemcpyDeviceToDevice, s));
     return GRAY_OK;
 }
 

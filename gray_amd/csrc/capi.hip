@@ -57,7 +57,8 @@ extern "C" void gray_ctx_destroy(gray_ctx *ctx) {
                     ctx->range_lo, ctx->range_hi, ctx->sorted_ids, ctx->keys,     ctx->keys_alt,   ctx->vals_alt,
                     ctx->hist,     ctx->block_sums, ctx->bounds_partial, ctx->bounds, ctx->prefix, ctx->suffix,
                     ctx->sparse,   ctx->top_nodes, ctx->top_count, ctx->err,    ctx->list_rec,   ctx->list_n,
-                    ctx->tasks[0], ctx->tasks[1], ctx->task_counters, ctx->need_ends, ctx->node_info};
+                    ctx->tasks[0], ctx->tasks[1], ctx->task_counters, ctx->need_ends, ctx->node_info,
+                    ctx->wide,     ctx->wide_flag};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

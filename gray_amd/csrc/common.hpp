@@ -45,6 +45,19 @@ struct __attribute__((aligned(16))) BvhNode {
     uint32_t pad0, pad1;
 };
 
+// 4-wide node of the collect pass (64 B: four 16-byte loads).  It collapses a
+// binary node at even depth with its internal children: the children are the
+// grandchildren reached through internal children and the node's own leaf
+// children.  Child boxes are stored by coordinate as half-precision pairs
+// rounded outward (min toward -inf, max toward +inf), so the cull stays
+// conservative; an unused child has ref GRAY_EMPTY and an empty box.
+struct __attribute__((aligned(64))) Bvh4Node {
+    uint32_t lo[3][2];  // min x, y, z: children (0,1) and (2,3) as half pairs, child 0/2 in the low half
+    uint32_t hi[3][2];  // max x, y, z
+    uint32_t child[4];
+};
+static_assert(sizeof(Bvh4Node) == 64, "Bvh4Node is 64 bytes");
+
 // Device copy of gray_knobs (device pointers to 1-element tensors).
 struct DevKnobs {
     const float *exp_power, *alpha_threshold, *t_threshold, *gsf, *eps_fwd, *eps_scale_grad, *bg;

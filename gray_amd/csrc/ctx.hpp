@@ -16,6 +16,8 @@ struct gray_ctx {
     float *aabb = nullptr;  // [cap,6]
     // LBVH (Karras layout: internal nodes 0..n-2, root 0)
     BvhNode *nodes = nullptr;       // [max(cap-1,1)]
+    Bvh4Node *wide = nullptr;       // [cap]: 4-wide nodes at the binary index of every even-depth internal node
+    uint8_t *wide_flag = nullptr;   // [cap]: 1 where an internal node lies at even depth (root: depth 0)
     uint32_t *parent_int = nullptr; // [cap] parent | side<<31 (root: GRAY_EMPTY)
     uint32_t *parent_leaf = nullptr;// [cap]
     uint32_t *range_lo = nullptr;   // [cap]

@@ -42,17 +42,22 @@ struct LaneRay {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
 };
 
-__device__ __forceinline__ bool slab_test(const LaneRay &r, const float *b, float tlo) {
-    const float tx0 = (b[0] - r.ox) * r.ix, tx1 = (b[3] - r.ox) * r.ix;
-    const float ty0 = (b[1] - r.oy) * r.iy, ty1 = (b[4] - r.oy) * r.iy;
-    const float tz0 = (b[2] - r.oz) * r.iz, tz1 = (b[5] - r.oz) * r.iz;
+__device__ __forceinline__ bool slab_hit(const LaneRay &r, float x0, float y0, float z0, float x1, float y1, float z1,
+                                         float tlo) {
+    const float tx0 = (x0 - r.ox) * r.ix, tx1 = (x1 - r.ox) * r.ix;
+    const float ty0 = (y0 - r.oy) * r.iy, ty1 = (y1 - r.oy) * r.iy;
+    const float tz0 = (z0 - r.oz) * r.iz, tz1 = (z1 - r.oz) * r.iz;
     const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
     const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
     // relative slack keeps the cull conservative under rounding; clamped so
     // an infinite slab bound (a zero direction component) cannot make it
     // infinite and accept every box
     const float slack = 1e-5f * fminf(fmaxf(fabsf(tn), fabsf(tf)), 1e30f) + 1e-7f;
-    return (b[0] <= b[3]) && (tn <= tf + slack) && (tf + slack >= tlo);
+    return (x0 <= x1) && (tn <= tf + slack) && (tf + slack >= tlo);
+}
+
+__device__ __forceinline__ bool slab_test(const LaneRay &r, const float *b, float tlo) {
+    return slab_hit(r, b[0], b[1], b[2], b[3], b[4], b[5], tlo);
 }
 
 // XCD-aware block -> tile mapping: workgroup b runs on XCD (b % 8); give each
@@ -148,6 +153,33 @@ __device__ __forceinline__ NodeV load_node(const BvhNode *__restrict__ nodes, ui
     n.left = __float_as_uint(d.x);
     n.right = __float_as_uint(d.y);
     return n;
+}
+
+// A 4-wide node is kept as loaded (16 VGPRs) across the loop and decoded
+// where its boxes are tested.
+struct Node4V {
+    uint4 a, b, c, ch;
+};
+
+__device__ __forceinline__ Node4V load_node4(const Bvh4Node *__restrict__ wide, uint32_t idx) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(wide + in_vgpr(idx));
+    Node4V n;
+    n.a = q[0], n.b = q[1], n.c = q[2], n.ch = q[3];
+    return n;
+}
+
+__device__ __forceinline__ float half_lo(uint32_t v) { return __half2float(__ushort_as_half((unsigned short)(v & 0xffffu))); }
+__device__ __forceinline__ float half_hi(uint32_t v) { return __half2float(__ushort_as_half((unsigned short)(v >> 16))); }
+
+// Child k's box test; the box words of child k are word k/2 of each pair, its
+// half is k%2.
+template <int k>
+__device__ __forceinline__ bool child_hit(const LaneRay &r, const Node4V &n) {
+    constexpr bool h = (k & 1) != 0;
+    const uint32_t lx = k < 2 ? n.a.x : n.a.y, ly = k < 2 ? n.a.z : n.a.w, lz = k < 2 ? n.b.x : n.b.y;
+    const uint32_t hx = k < 2 ? n.b.z : n.b.w, hy = k < 2 ? n.c.x : n.c.y, hz = k < 2 ? n.c.z : n.c.w;
+    auto f = [](uint32_t v) { return h ? half_hi(v) : half_lo(v); };
+    return slab_hit(r, f(lx), f(ly), f(lz), f(hx), f(hy), f(hz), 0.0f);
 }
 
 __device__ __forceinline__ GeoV load_geo(const GeoRecord *__restrict__ geo, uint32_t gid) {
@@ -347,10 +379,10 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 // Iterations a walk runs before it hands its remaining work on as tasks, in
 // the per-tile pass and in every task round but the last (which finishes).
 #ifndef GRAY_COLLECT_BUDGET
-#define GRAY_COLLECT_BUDGET 384
+#define GRAY_COLLECT_BUDGET 192
 #endif
 #ifndef GRAY_TASK_BUDGET
-#define GRAY_TASK_BUDGET 192
+#define GRAY_TASK_BUDGET 96
 #endif
 #ifndef GRAY_TASK_ROUNDS
 #define GRAY_TASK_ROUNDS 3
@@ -377,6 +409,30 @@ __device__ __forceinline__ float fp16_round(float x, bool on) { return on ? __ha
 __device__ __forceinline__ void put_hit(const TraceArgs &A, size_t g, int slot, float dist, uint32_t gid, float a) {
     const size_t e = (size_t)slot * (size_t)A.list_stride + g;
     A.list_rec[e] = make_float4(dist, __uint_as_float(gid), a, 0.0f);
+}
+
+// Exact test of leaf gid for the lane's ray; a hit goes to the ray's list at
+// slot cnt (the per-tile pass owns the ray) or at a slot taken from list_n[g]
+// with an atomic (tasks of one ray run concurrently).  ok turns false when the
+// list overflows.
+template <bool kTask>
+__device__ __forceinline__ void collect_leaf(const TraceArgs &A, const GeoV &q, uint32_t gid, const LaneRay &r,
+                                             size_t g, int cap, float athr, float p, int &cnt, bool &ok) {
+    if (q.op <= athr) return;
+    float dist, hx, hy, hz;
+    if (!gray_hit_test(q.m, q.mu[0], q.mu[1], q.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz)) return;
+    float gv, qq;
+    const float a = gray_alpha(hx, hy, hz, q.clip, q.op, p, gv, qq);
+    dist = fp16_round(dist, A.fp16 != 0);
+    if (kTask) {
+        const int slot = atomicAdd(A.list_n + g, 1);
+        if (slot >= 0 && slot < cap) put_hit(A, g, slot, dist, gid, a);
+        else ok = false;
+    } else {
+        if (cnt < cap) put_hit(A, g, cnt, dist, gid, a);
+        ++cnt;
+        if (cnt > cap) ok = false;
+    }
 }
 
 // One lane's walk of the LBVH for its ray (the order of the walk does not
@@ -461,23 +517,7 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
         for (int l = 0; l < NL; ++l) {
             if (pl[l] == GRAY_EMPTY) continue;
             ++leaves;
-            const GeoV &q = pg[l];
-            if (q.op <= athr) continue;
-            float dist, hx, hy, hz;
-            if (!gray_hit_test(q.m, q.mu[0], q.mu[1], q.mu[2], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz))
-                continue;
-            float gv, qq;
-            const float a = gray_alpha(hx, hy, hz, q.clip, q.op, p, gv, qq);
-            dist = fp16_round(dist, A.fp16 != 0);
-            if (kTask) {
-                const int slot = atomicAdd(A.list_n + g, 1);
-                if (slot >= 0 && slot < cap) put_hit(A, g, slot, dist, pl[l], a);
-                else ok = false;
-            } else {
-                if (cnt < cap) put_hit(A, g, cnt, dist, pl[l], a);
-                ++cnt;
-                if (cnt > cap) ok = false;
-            }
+            collect_leaf<kTask>(A, pg[l], pl[l], r, g, cap, athr, p, cnt, ok);
         }
         uint32_t nh[NH];
 #pragma unroll
@@ -551,9 +591,148 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
     }
 }
 
+// The same walk over the 4-wide nodes, one node per iteration: a visit tests
+// the four child boxes; hit leaves join the queue, the first hit internal
+// child is the next node and the others go onto the stack (or, when it is
+// full, become tasks).  Node visits pause while the queue could not take four
+// leaves.  References are binary node indices (of even depth), so tasks and
+// the root are shared with the binary walk.
+template <bool kTask, int kStack>
+__device__ __forceinline__ void collect_walk_wide(const TraceArgs &A, const GeoRecord *__restrict__ geo,
+                                                  const Bvh4Node *__restrict__ wide, const LaneRay &r, size_t g,
+                                                  uint32_t start, int budget, uint32_t *__restrict__ s_stack,
+                                                  uint32_t *__restrict__ s_leaf, int lane, int &cnt, bool &ok,
+                                                  int &visits, int &leaves) {
+    constexpr int NL = kCollectLeaves;
+    const int cap = A.list_cap;
+    const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power;
+    uint32_t h = GRAY_EMPTY;
+    Node4V nd;
+    uint32_t pl[NL];
+    GeoV pg[NL];
+#pragma unroll
+    for (int l = 0; l < NL; ++l) pl[l] = GRAY_EMPTY;
+    if (start != GRAY_EMPTY) {
+        if (start & GRAY_LEAF) {
+            pl[0] = start & ~GRAY_LEAF;
+            pg[0] = load_geo(geo, pl[0]);
+        } else {
+            h = start;
+            nd = load_node4(wide, start);
+        }
+    }
+    int sp = 0, qh = 0, qn = 0;
+    for (int it = 0;; ++it) {
+        if (!__ballot(h != GRAY_EMPTY || pl[0] != GRAY_EMPTY)) break;
+        if (it == budget) {
+            int m = sp + qn + (h != GRAY_EMPTY);
+#pragma unroll
+            for (int l = 0; l < NL; ++l) m += pl[l] != GRAY_EMPTY;
+            if (m > 0) {
+                // entries past the buffer's end are dropped and the ray is
+                // re-traversed; the ones that fit are still written
+                const int base = atomicAdd(A.task_count_out, m);
+                if (base + m > A.task_cap) ok = false;
+                const int end = min(base + m, A.task_cap);
+                int o = base;
+                auto put = [&](uint32_t ref) {
+                    if (o < end) A.task_out[o] = make_uint2((uint32_t)g, ref);
+                    ++o;
+                };
+                if (h != GRAY_EMPTY) put(h);
+#pragma unroll
+                for (int l = 0; l < NL; ++l)
+                    if (pl[l] != GRAY_EMPTY) put(pl[l] | GRAY_LEAF);
+                for (int k = 0; k < sp; ++k) put(s_stack[k * GRAY_WAVE + lane]);
+                for (int k = 0; k < qn; ++k) put(s_leaf[((qh + k) & (kLeafQueue - 1)) * GRAY_WAVE + lane] | GRAY_LEAF);
+            }
+            break;
+        }
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            if (pl[l] == GRAY_EMPTY) continue;
+            ++leaves;
+            collect_leaf<kTask>(A, pg[l], pl[l], r, g, cap, athr, p, cnt, ok);
+        }
+        uint32_t nh = h;
+        bool moved = false;
+        if (h != GRAY_EMPTY && qn <= kLeafQueue - 4) {
+            moved = true;
+            ++visits;
+            uint32_t ch[4];
+            ch[0] = (nd.ch.x != GRAY_EMPTY && child_hit<0>(r, nd)) ? nd.ch.x : GRAY_EMPTY;
+            ch[1] = (nd.ch.y != GRAY_EMPTY && child_hit<1>(r, nd)) ? nd.ch.y : GRAY_EMPTY;
+            ch[2] = (nd.ch.z != GRAY_EMPTY && child_hit<2>(r, nd)) ? nd.ch.z : GRAY_EMPTY;
+            ch[3] = (nd.ch.w != GRAY_EMPTY && child_hit<3>(r, nd)) ? nd.ch.w : GRAY_EMPTY;
+            nh = GRAY_EMPTY;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t c = ch[k];
+                if (c == GRAY_EMPTY) continue;
+                if (c & GRAY_LEAF) {
+                    s_leaf[((qh + qn) & (kLeafQueue - 1)) * GRAY_WAVE + lane] = c & ~GRAY_LEAF;
+                    ++qn;
+                } else if (nh == GRAY_EMPTY) {
+                    nh = c;
+                } else if (sp < kStack) {
+                    s_stack[sp * GRAY_WAVE + lane] = c;
+                    ++sp;
+                } else if (budget >= 0) {
+                    const int o = atomicAdd(A.task_count_out, 1);
+                    if (o < A.task_cap) A.task_out[o] = make_uint2((uint32_t)g, c);
+                    else ok = false;
+                } else {
+                    ok = false;
+                }
+            }
+            if (nh == GRAY_EMPTY && sp > 0) {
+                --sp;
+                nh = s_stack[sp * GRAY_WAVE + lane];
+            }
+        }
+        if (!ok) {
+            nh = GRAY_EMPTY;
+            sp = qn = 0;
+        }
+        // requests: the next leaf records first, then the next node
+#pragma unroll
+        for (int l = 0; l < NL; ++l) {
+            pl[l] = GRAY_EMPTY;
+            if (qn > 0) {
+                pl[l] = s_leaf[qh * GRAY_WAVE + lane];
+                qh = (qh + 1) & (kLeafQueue - 1);
+                --qn;
+                pg[l] = load_geo(geo, pl[l]);
+            }
+        }
+        h = nh;
+        if (moved && h != GRAY_EMPTY) nd = load_node4(wide, h);
+    }
+}
+
+// Which tree the collect pass walks: the 4-wide collapse (1) or the binary
+// LBVH (0).  Both give the same hit lists; they differ in speed.
+#ifndef GRAY_COLLECT_WIDE
+#define GRAY_COLLECT_WIDE 1
+#endif
+#if GRAY_COLLECT_WIDE
+using CollectNode = Bvh4Node;
+static_assert(kLeafQueue >= 4, "a 4-wide visit may queue four leaves");
+template <bool kTask, int kStack, typename... T>
+__device__ __forceinline__ void collect_any(T &&...a) {
+    collect_walk_wide<kTask, kStack>(static_cast<T &&>(a)...);
+}
+#else
+using CollectNode = BvhNode;
+template <bool kTask, int kStack, typename... T>
+__device__ __forceinline__ void collect_any(T &&...a) {
+    collect_walk<kTask, kStack>(static_cast<T &&>(a)...);
+}
+#endif
+
 // Per-tile collect pass: every ray of the tile walks from the root.
 __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRecord *__restrict__ geo,
-                                                       const BvhNode *__restrict__ nodes) {
+                                                       const CollectNode *__restrict__ nodes) {
     __shared__ uint32_t s_stack[kCollectStack * GRAY_WAVE];
     __shared__ uint32_t s_leaf[kLeafQueue * GRAY_WAVE];
     int vi, tx, ty, tile;
@@ -567,8 +746,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
     const size_t g = (size_t)tile * GRAY_WAVE + lane;
     int cnt = 0, visits = 0, leaves = 0;
     bool ok = true;
-    collect_walk<false, kCollectStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
-                                       A.budget, s_stack, s_leaf, lane, cnt, ok, visits, leaves);
+    collect_any<false, kCollectStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
+                                      A.budget, s_stack, s_leaf, lane, cnt, ok, visits, leaves);
     A.list_n[g] = ok ? cnt : -1;
     if (A.tile_stats) {
         int hsum = ok ? cnt : 0;
@@ -607,7 +786,7 @@ __device__ __forceinline__ const DevView &lane_pixel(const TraceArgs &A, uint32_
 // passes it.
 template <int kStack>
 __global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const GeoRecord *__restrict__ geo,
-                                                             const BvhNode *__restrict__ nodes) {
+                                                             const CollectNode *__restrict__ nodes) {
     __shared__ uint32_t s_stack[kStack * GRAY_WAVE];
     __shared__ uint32_t s_leaf[kLeafQueue * GRAY_WAVE];
     const int lane = threadIdx.x;
@@ -632,8 +811,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const 
         }
         int cnt = 0, visits = 0, leaves = 0;
         bool ok = true;
-        collect_walk<true, kStack>(A, geo, nodes, r, g, start, A.budget, s_stack, s_leaf, lane, cnt, ok, visits,
-                                   leaves);
+        collect_any<true, kStack>(A, geo, nodes, r, g, start, A.budget, s_stack, s_leaf, lane, cnt, ok, visits,
+                                  leaves);
         if (!ok && start != GRAY_EMPTY) atomicExch(A.list_n + g, kRayDead);
     }
 }
@@ -1807,7 +1986,12 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             A.task_count_out = count;
             A.budget = args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
             if ((rc = mark(0))) return rc;
-            hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, A.nodes);
+#if GRAY_COLLECT_WIDE
+            const CollectNode *cnodes = ctx->wide;
+#else
+            const CollectNode *cnodes = ctx->nodes;
+#endif
+            hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, cnodes);
             GRAY_HIP(hipGetLastError());
             if ((rc = mark(1))) return rc;
             for (int round = 0; round < kTaskRounds; ++round) {
@@ -1820,10 +2004,10 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
                                                    : -1;
                 if (A.budget >= 0)
                     hipLaunchKernelGGL(k_collect_tasks<kCollectStack>, dim3(kTaskWaves), block, 0, s, A, A.geo,
-                                       A.nodes);
+                                       cnodes);
                 else
                     hipLaunchKernelGGL(k_collect_tasks<kFinalStack>, dim3(kTaskWaves), block, 0, s, A, A.geo,
-                                       A.nodes);
+                                       cnodes);
                 GRAY_HIP(hipGetLastError());
             }
             if ((rc = mark(2))) return rc;

@@ -586,6 +586,16 @@ class NativeRaytracer:
                                      self._stream()))
         return nodes, ids, keys
 
+    def debug_wide(self):
+        """4-wide nodes of the collect pass at the binary node index: nodes
+        [N-1,16] int32 (half-pair boxes, child refs) and flags [N-1] uint8 (1 =
+        a 4-wide node: internal node at even depth)."""
+        nn = max(self.gaussian_data.count - 1, 1)
+        nodes = torch.zeros(nn, 16, dtype=torch.int32, device=self.device)
+        flags = torch.zeros(nn, dtype=torch.uint8, device=self.device)
+        L.check(L.LIB.gray_debug_wide(self.ctx.handle, nodes.data_ptr(), flags.data_ptr(), self._stream()))
+        return nodes, flags
+
     def set_device_timing(self, enable: bool) -> None:
         """Record HIP events around BVH rebuilds and the forward's stages."""
         L.check(L.LIB.gray_bvh_set_timing(self.ctx.handle, 1 if enable else 0))

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 6
+#define GRAY_ABI_VERSION 7
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -341,6 +341,15 @@ int gray_debug_snapshot(gray_ctx *ctx, float *geo, float *aabb, void *stream);
  * as uint32 bits (bit 31 = leaf, payload Gaussian id; 0xffffffff = none),
  * sorted_ids [N] (leaf order), keys [N] (30-bit Morton codes, ascending). */
 int gray_debug_bvh(gray_ctx *ctx, float *nodes, uint32_t *sorted_ids, uint32_t *keys, void *stream);
+
+/* No reference counterpart (diagnostic of the collect pass's tree).  Copies
+ * the 4-wide nodes of the last gray_bvh_update (device buffers): nodes
+ * [max(N-1,1), 16] uint32 at the binary node index = {min x, min y, min z,
+ * max x, max y, max z as half pairs (children 0,1 then 2,3; the lower-numbered
+ * child in the low half), child refs 4}, meaningful where flags [max(N-1,1)]
+ * is 1 (internal nodes at even depth, root depth 0).  Either pointer may be
+ * NULL. */
+int gray_debug_wide(gray_ctx *ctx, uint32_t *nodes, uint8_t *flags, void *stream);
 
 /* No reference counterpart (diagnostic of the collect pass).  Copies the hit
  * counts per ray of the last forward launch that collected lists (device
