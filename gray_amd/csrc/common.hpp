@@ -135,6 +135,11 @@ struct TraceArgs {
     int32_t *task_count_in, *task_count_out, *task_head;
     int32_t task_cap;
     int32_t fp16;  // fp16_hits: distances and compositing alphas rounded to fp16 as the reference stores them
+    // rays whose list is composited by a whole wave: lists longer than
+    // wave_list_min (< 0: none), queued by the per-lane compositing kernel
+    int32_t *heavy_count;
+    uint32_t *heavy_rays;
+    int32_t wave_list_min, pad5;
 };
 
 // ---------------------------------------------------------------------------

@@ -1088,6 +1088,11 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
 // ---------------------------------------------------------------------------
 constexpr uint32_t kListMark = 0x80000000u;  // list id flag: entry was composited
 constexpr int kListBatch = 8;                 // list entries loaded per step of a scan
+constexpr int kWavePer = 8;                   // list entries per lane of the wave-per-ray kernel
+constexpr int kWaveListMax = GRAY_WAVE * kWavePer;
+#ifndef GRAY_WAVE_LIST_MIN
+#define GRAY_WAVE_LIST_MIN 96
+#endif
 
 // The GRAY_KB nearest entries of a scan, sorted by (distance, id), held in
 // registers (every index is a compile-time constant after unrolling).  Empty
@@ -1146,6 +1151,18 @@ __global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) 
     const size_t lg = (size_t)tile * GRAY_WAVE + lane;
     const int n = A.list_n[lg];
     active = active && n >= 0;
+    // long lists go to the wave-per-ray kernel (one queue slot per lane,
+    // taken with one atomic per wave)
+    const bool heavy = active && A.wave_list_min >= 0 && n > A.wave_list_min && n <= kWaveListMax;
+    const uint64_t hb = __ballot(heavy);
+    if (hb) {
+        const int first = __ffsll((long long)hb) - 1;
+        int base = 0;
+        if (lane == first) base = atomicAdd(A.heavy_count, __popcll(hb));
+        base = __shfl(base, first);
+        if (heavy) A.heavy_rays[base + __popcll(hb & ((1ull << lane) - 1ull))] = (uint32_t)lg;
+    }
+    active = active && !heavy;
     if (!__ballot(active)) return;
 
     const size_t S = (size_t)A.list_stride;
@@ -1297,6 +1314,223 @@ __global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) 
     if (A.tile_stats && lane == 0) A.tile_stats[4 * (size_t)(A.tile_base_launch + tile) + 2] = nrounds;
     if (!active) return;
     store_pixel(A, V, pix, r, depth_on, T, Tf, pnc, skc0, skc1, skc2, ska, C0, C1, C2, D, ncomp, (uint32_t)nskip);
+}
+
+// ---------------------------------------------------------------------------
+// Forward from long lists: one wave per ray
+// ---------------------------------------------------------------------------
+// Wave reductions with DPP: steps within 16-lane rows, row broadcasts across
+// them; lane 63 ends with the total, which is returned to every lane (the
+// tree, and so the rounding, is fixed).
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, kCtrl, kRowMask, 0xF, false);
+}
+
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __uint_as_float(dpp_u32<kCtrl, kRowMask>(__float_as_uint(v)));
+}
+
+// (d, id) < (d', id') without branches
+__device__ __forceinline__ bool key_less_nb(float d0, uint32_t i0, float d1, uint32_t i1) {
+    return (d0 < d1) | ((d0 == d1) & (i0 < i1));
+}
+
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ void dpp_min_step(float &d, uint32_t &id) {
+    const float od = dpp_f32<kCtrl, kRowMask>(d);
+    const uint32_t oi = dpp_u32<kCtrl, kRowMask>(id);
+    const bool take = key_less_nb(od, oi, d, id);
+    d = take ? od : d;
+    id = take ? oi : id;
+}
+
+// minimum key (d, id) over the wave
+__device__ __forceinline__ void wave_min_key(float &d, uint32_t &id) {
+    dpp_min_step<0xB1, 0xF>(d, id);   // quad_perm [1,0,3,2]
+    dpp_min_step<0x4E, 0xF>(d, id);   // quad_perm [2,3,0,1]
+    dpp_min_step<0x141, 0xF>(d, id);  // row_half_mirror
+    dpp_min_step<0x140, 0xF>(d, id);  // row_mirror
+    dpp_min_step<0x142, 0xA>(d, id);  // row_bcast:15 into rows 1, 3
+    dpp_min_step<0x143, 0xC>(d, id);  // row_bcast:31 into rows 2, 3
+    d = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), 63));
+    id = (uint32_t)__builtin_amdgcn_readlane((int)id, 63);
+}
+
+template <bool kProd>
+__device__ __forceinline__ float wave_reduce(float v) {
+    auto op = [](float a, float b) { return kProd ? a * b : a + b; };
+    v = op(v, dpp_f32<0xB1, 0xF>(v));
+    v = op(v, dpp_f32<0x4E, 0xF>(v));
+    v = op(v, dpp_f32<0x141, 0xF>(v));
+    v = op(v, dpp_f32<0x140, 0xF>(v));
+    v = op(v, dpp_f32<0x142, 0xA>(v));
+    v = op(v, dpp_f32<0x143, 0xC>(v));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// The per-lane kernel's batch semantics for one ray per wave.  Each lane holds
+// up to kWavePer of the ray's list entries, their keys (distance, id) sorted in
+// registers and their alpha and colour in LDS; the wave takes the nearest
+// remaining entry (a wave minimum over the lanes' first keys) one at a time
+// and composites it exactly as the per-lane kernel does, so composited hits,
+// transmittances and colour sums are the same operations in the same order.
+// The batch rules follow from the sorted order: a round of GRAY_KB selects the
+// next entries, and after every GRAY_BATCH selected entries those at the same
+// distance are dropped (strict cursor).  The skip set (entries not composited)
+// is summed per lane and then over the wave.  Rays come from the queue the
+// per-lane kernel filled.
+__global__ __launch_bounds__(GRAY_WAVE) void k_composite_wave(TraceArgs A, const float *__restrict__ rgb0) {
+    constexpr int P = kWavePer;
+    __shared__ float4 stage[P][GRAY_WAVE];  // {alpha (fp32), colour} of entry j of each lane
+    const int lane = threadIdx.x;
+    const int total = *A.heavy_count;
+    const size_t S = (size_t)A.list_stride;
+    const float4 *__restrict__ L = A.list_rec;
+    const float tthr = *A.knobs.t_threshold;
+    const bool depth_on = *A.knobs.render_depth != 0;
+    const bool h16 = A.fp16 != 0;
+    for (int q = blockIdx.x; q < total; q += gridDim.x) {
+        const uint32_t g = A.heavy_rays[q];
+        const int n = A.list_n[g];
+        float d[P], a[P];
+        uint32_t id[P], js[P];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const int i = lane + GRAY_WAVE * j;
+            d[j] = INFINITY;
+            id[j] = 0xFFFFFFFFu;
+            a[j] = 0.0f;
+            js[j] = (uint32_t)j;
+            if (i < n) {
+                const float4 rec = L[(size_t)i * S + g];
+                d[j] = rec.x;
+                id[j] = __float_as_uint(rec.y) & ~kListMark;
+                a[j] = rec.z;
+            }
+        }
+        int x, y;
+        const DevView &V = lane_pixel(A, g, x, y);
+        const float *__restrict__ rgb = rgb0 + (V.rgb - A.views[0].rgb);
+        // colours in flight while the keys are sorted
+        float c[P][3];
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            c[j][0] = c[j][1] = c[j][2] = 0.0f;
+            if (lane + GRAY_WAVE * j < n) {
+                c[j][0] = rgb[3 * id[j] + 0];
+                c[j][1] = rgb[3 * id[j] + 1];
+                c[j][2] = rgb[3 * id[j] + 2];
+            }
+        }
+        // 19-comparator sorting network for 8 (padding sorts last); the
+        // original index travels with the key
+        auto cx = [&](int i, int k) {
+            const bool sw = key_less_nb(d[k], id[k], d[i], id[i]);
+            const float td = d[i];
+            const uint32_t ti = id[i], tj = js[i];
+            d[i] = sw ? d[k] : d[i], id[i] = sw ? id[k] : id[i], js[i] = sw ? js[k] : js[i];
+            d[k] = sw ? td : d[k], id[k] = sw ? ti : id[k], js[k] = sw ? tj : js[k];
+        };
+        static_assert(P == 8, "sorting network for 8 entries per lane");
+        cx(0, 2), cx(1, 3), cx(4, 6), cx(5, 7);
+        cx(0, 4), cx(1, 5), cx(2, 6), cx(3, 7);
+        cx(0, 1), cx(2, 3), cx(4, 5), cx(6, 7);
+        cx(2, 4), cx(3, 5);
+        cx(1, 4), cx(3, 6);
+        cx(1, 2), cx(3, 4), cx(5, 6);
+#pragma unroll
+        for (int j = 0; j < P; ++j) stage[j][lane] = make_float4(a[j], c[j][0], c[j][1], c[j][2]);
+        __syncthreads();
+        LaneRay r{};
+        make_ray(A.knobs, V, x, y, r);
+        const float znear = *V.znear, zfar = *V.zfar;
+        const int pix = y * V.stride + x;
+        const int R = V.ray_count;
+        int cnt = n > lane ? min(P, (n - lane + GRAY_WAVE - 1) / GRAY_WAVE) : 0;
+        float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, sa = 0.0f, sp = 1.0f;
+        auto skip_add = [&](uint32_t j) {
+            const float4 e = stage[j][lane];
+            if (e.x == 0.0f) return;
+            const float wh = fp16_round(e.x, h16);  // as the per-lane kernel's skip scan
+            s0 += wh * e.y;
+            s1 += wh * e.z;
+            s2 += wh * e.w;
+            sa += wh;
+            sp = sp * (1.0f - e.x);
+        };
+        auto pop = [&]() {
+#pragma unroll
+            for (int j = 0; j + 1 < P; ++j) d[j] = d[j + 1], id[j] = id[j + 1], js[j] = js[j + 1];
+            d[P - 1] = INFINITY, id[P - 1] = 0xFFFFFFFFu, js[P - 1] = 0u;
+            --cnt;
+        };
+        float T = 1.0f, Tf = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f, tmin = znear;
+        uint32_t ncomp = 0;
+        const bool traced = tmin < zfar;
+        if (traced) {
+            // the first cursor is strict: entries at or before znear are never selected
+            for (int j = 0; j < P && cnt > 0 && !(d[0] > znear); ++j) {
+                skip_add(js[0]);
+                pop();
+            }
+            int in_round = 0, bpos = 0, batch = 0;
+            for (int step = 0; step < n; ++step) {
+                float md = cnt > 0 ? d[0] : INFINITY;
+                uint32_t mi = cnt > 0 ? id[0] : 0xFFFFFFFFu;
+                wave_min_key(md, mi);
+                if (mi == 0xFFFFFFFFu) break;  // exhausted: the last round was partial
+                const uint64_t own = __ballot(cnt > 0 && id[0] == mi);
+                const int owner = __ffsll((long long)own) - 1;
+                const uint32_t oj = (uint32_t)__builtin_amdgcn_readlane((int)js[0], owner);
+                const float4 e = stage[oj][owner];
+                tmin = fmaxf(md, tmin);
+                if (!(md < zfar)) break;  // this entry and every later one stay in the skip set
+                const float a32 = e.x;
+                const float av = fp16_round(a32, h16);
+                const float w = T * av;
+                C0 += e.y * w;
+                C1 += e.z * w;
+                C2 += e.w * w;
+                if (depth_on) D += md * w;
+                if (lane == 0 && A.save && (int)ncomp < A.capacity) {
+                    V.hit_ids[(size_t)ncomp * R + pix] = mi;
+                    V.hit_T[(size_t)ncomp * R + pix] = T;
+                }
+                T = T * (1.0f - av);
+                Tf = Tf * (1.0f - a32);
+                ++ncomp;
+                if (lane == owner) pop();
+                if (T < tthr) break;
+                if (++in_round == GRAY_KB) {
+                    in_round = 0;
+                    bpos += GRAY_KB;
+                    if (bpos >= GRAY_BATCH) {
+                        bpos = 0;
+                        ++batch;
+                        for (int j = 0; j < P && cnt > 0 && !(d[0] > tmin); ++j) {
+                            skip_add(js[0]);
+                            pop();
+                        }
+                        if (batch >= GRAY_MAX_ROUNDS) break;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < P; ++j)
+                if (j < cnt) skip_add(js[j]);
+        }
+        s0 = wave_reduce<false>(s0);
+        s1 = wave_reduce<false>(s1);
+        s2 = wave_reduce<false>(s2);
+        sa = wave_reduce<false>(sa);
+        sp = wave_reduce<true>(sp);
+        if (lane == 0)
+            store_pixel(A, V, pix, r, depth_on, T, Tf, sp, s0, s1, s2, sa, C0, C1, C2, D, ncomp,
+                        traced ? (uint32_t)n : 0u);
+        __syncthreads();  // the next ray overwrites the stage
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1869,12 +2103,19 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
     A.nodes = ctx->nodes;
     A.l2w = ctx->l2w;
     A.error_flag = ctx->err;
+    A.heavy_count = nullptr;
+    A.heavy_rays = nullptr;
+    A.wave_list_min = -1;
     return GRAY_OK;
 }
 
 // Grows the collect-pass lists to lanes * cap records and the task buffers
 // to kTaskPerLane entries per lane (never shrinks).
 constexpr int64_t kTaskPerLane = 16;
+// device counters of a launch: tasks handed to each round, each round's take
+// position, the wave-per-ray queue length and a spare
+constexpr int kCounters = 2 * kTaskRounds + 4;
+constexpr int kWaveGrid = 8192;  // persistent waves of the wave-per-ray kernel
 
 int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
     if (cap > (1 << 16)) {
@@ -1892,8 +2133,8 @@ int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
         return GRAY_ERR_NOMEM;
     };
     if (!ctx->task_counters) {
-        if (hipMalloc(&ctx->task_counters, sizeof(int32_t) * (2 * kTaskRounds + 2)) != hipSuccess)
-            return fail("task counters", sizeof(int32_t) * (2 * kTaskRounds + 2));
+        if (hipMalloc(&ctx->task_counters, sizeof(int32_t) * kCounters) != hipSuccess)
+            return fail("task counters", sizeof(int32_t) * kCounters);
     }
     if (lanes > ctx->list_lanes_cap) {
         GRAY_HIP(hipFree(ctx->list_n));
@@ -1980,7 +2221,7 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             ctx->list_lanes = lanes;
             // counters: count[r] tasks handed to round r, head[r] its take position
             int32_t *count = ctx->task_counters, *head = ctx->task_counters + kTaskRounds + 1;
-            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, ctx->task_counters, 2 * kTaskRounds + 2);
+            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, s, ctx->task_counters, kCounters);
             GRAY_HIP(hipGetLastError());
             A.task_out = ctx->tasks[0];
             A.task_count_out = count;
@@ -2011,8 +2252,17 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
                 GRAY_HIP(hipGetLastError());
             }
             if ((rc = mark(2))) return rc;
+            // the task buffers are free once the last round has run: the first
+            // holds the queue of rays composited by a whole wave (one entry per lane)
+            A.heavy_count = ctx->task_counters + 2 * kTaskRounds + 2;
+            A.heavy_rays = reinterpret_cast<uint32_t *>(ctx->tasks[0]);
+            A.wave_list_min = args->wave_list_min == 0 ? GRAY_WAVE_LIST_MIN : args->wave_list_min;
             hipLaunchKernelGGL(k_composite_lists, grid, block, 0, s, A, A.views[0].rgb);
             GRAY_HIP(hipGetLastError());
+            if (A.wave_list_min >= 0) {
+                hipLaunchKernelGGL(k_composite_wave, dim3(kWaveGrid), block, 0, s, A, A.views[0].rgb);
+                GRAY_HIP(hipGetLastError());
+            }
         } else {
             for (int i = 0; i < 3; ++i)
                 if ((rc = mark(i))) return rc;

@@ -324,6 +324,7 @@ class NativeRaytracer:
         # the reference's fp16 storage of hit distances and alphas
         # (per_pixel_linked_list.h:28-29); False: fp32 throughout
         self.fp16_hits = False
+        self.wave_list_min = 0  # lists longer than this are composited by a whole wave (0: built-in, < 0: never)
         self.tile_stats = None  # optional int32 [tiles, 4] traversal statistics of the forward
         self.timer = _NoTimer()
         self._steps = 0
@@ -385,6 +386,7 @@ class NativeRaytracer:
         a.list_capacity = int(self.list_capacity)
         a.collect_budget = int(self.collect_budget)
         a.fp16_hits = 1 if self.fp16_hits else 0
+        a.wave_list_min = int(self.wave_list_min)
         return a
 
     def _require_bvh(self) -> None:

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 7
+#define GRAY_ABI_VERSION 8
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -137,7 +137,10 @@ typedef struct {
                                    hit distances and the alphas used for compositing, depth and the
                                    truncation estimate rounded to fp16 (round to nearest even); the full
                                    transmittance keeps the fp32 alpha (shaders.cu).  0 = fp32 throughout */
-    int32_t pad;
+    int32_t wave_list_min;      /* collected lists longer than this (and at most 512 hits) are composited by
+                                   a whole wave each instead of one lane: same composited hits, transmittance
+                                   and colour; the skip-set sums are added in another order.  0 = the
+                                   built-in default (96), < 0 = never */
 } gray_render_args;
 
 /* Gradient targets of the backward (reference gaussians.h:34-62). */

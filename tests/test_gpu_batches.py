@@ -1,7 +1,12 @@
 """Reference batch semantics on the GPU (forward_pass.cu:150-230): batches of
 32 hits strictly beyond tmin, served here in sub-rounds of 16.  Exact distance
 ties inside a batch are both composited; at a batch boundary the later one is
-dropped.  Long lists exercise the re-traversal (recompute) backward."""
+dropped.  Long lists exercise the re-traversal (recompute) backward.
+
+The 80-hit lists are composited by a whole wave per ray by default and by one
+lane per ray in the "lists-lane" case; the two kernels are also compared with
+each other on a larger scene (composited hits, saved lists and
+transmittances bit for bit)."""
 import numpy as np
 import pytest
 import torch
@@ -28,11 +33,12 @@ def axis_scene(dup_at=None):
     return mean, q, s, logit, colors
 
 
-def run(mean, q, s, logit, colors, tthr, capacity, list_capacity=512):
+def run(mean, q, s, logit, colors, tthr, capacity, list_capacity=512, wave_list_min=0):
     W = H = 8
     nr = NativeRaytracer(W, H, N_AXIS, 0, 1, 1, hit_capacity=capacity)
     # a negative capacity means: lists of that size, walks split into tasks every iteration
     nr.list_capacity = abs(list_capacity)
+    nr.wave_list_min = wave_list_min
     nr.collect_budget = 1 if list_capacity < 0 else 0
     g = nr.get_gaussians()
     cfg = nr.get_config()
@@ -59,12 +65,13 @@ def run(mean, q, s, logit, colors, tthr, capacity, list_capacity=512):
     return nr
 
 
-@pytest.mark.parametrize("list_capacity", [512, 0, 64, -512], ids=["lists", "traverse", "overflow", "tasks"])
+@pytest.mark.parametrize("list_capacity,wave", [(512, 0), (512, -1), (0, 0), (64, 0), (-512, 0)],
+                         ids=["lists", "lists-lane", "traverse", "overflow", "tasks"])
 @pytest.mark.parametrize("dup_at,tthr,expect_drop", [(None, 0.0, 0), (None, 0.05, 0), (15, 0.0, 0), (31, 0.0, 1),
                                                      (47, 0.0, 0), (63, 0.0, 1)])
-def test_batches_and_ties(dup_at, tthr, expect_drop, list_capacity):
+def test_batches_and_ties(dup_at, tthr, expect_drop, list_capacity, wave):
     mean, q, s, logit, colors = axis_scene(dup_at)
-    nr = run(mean, q, s, logit, colors, tthr, capacity=64, list_capacity=list_capacity)
+    nr = run(mean, q, s, logit, colors, tthr, capacity=64, list_capacity=list_capacity, wave_list_min=wave)
     fb = nr.get_framebuffer()
     ro = fb.ray_origin.reshape(-1, 3).cpu().numpy()
     rd = fb.ray_direction.reshape(-1, 3).cpu().numpy()
@@ -98,3 +105,45 @@ def test_recompute_backward_long_lists():
     for other in grads[1:]:
         for key in grads[0]:
             assert rel_err(other[key], grads[0][key]) < 1e-3, key
+
+
+@pytest.mark.parametrize("fp16,znear,zfar,tthr", [(False, 0.0, 99999.9, 0.03), (True, 0.0, 99999.9, 0.03),
+                                                  (False, 2.8, 3.3, 0.0), (False, 0.0, 99999.9, 0.0)])
+def test_wave_matches_lane(fp16, znear, zfar, tthr):
+    """Every list of two hits or more composited by a whole wave against the
+    per-lane kernel on the same snapshot: composited counts, saved lists,
+    transmittances, full transmittance and depth bit for bit; colour and
+    truncation estimate (skip-set sums in another order) within 1e-6."""
+    from gray_amd import scenes
+    from gpu_util import setup_native
+    scene = scenes.tiny_scene(20000, seed=5)
+    cam = scenes.tiny_camera(64, 48)
+    out = {}
+    for wave in (-1, 1):
+        nr = setup_native(scene, cam, t_threshold=tthr, znear=znear, zfar=zfar, hit_capacity=64,
+                          wave_list_min=wave)
+        nr.fp16_hits = fp16
+        nr.get_config().render_depth.fill_(True)
+        torch.set_grad_enabled(True)
+        nr.forward_pass()
+        torch.cuda.synchronize()
+        nr.device_status()
+        fb, h = nr.get_framebuffer(), nr.hits
+        out[wave] = dict(C=fb.output_channels.cpu().numpy(), T=fb.transmittance.cpu().numpy(),
+                         F=fb.full_transmittance.cpu().numpy(), D=fb.output_depth.cpu().numpy(),
+                         E=fb.remaining_channels_estimate.cpu().numpy(),
+                         n=h.n_comp.cpu().numpy(), ids=h.hit_ids.cpu().numpy(), Th=h.hit_trans.cpu().numpy(),
+                         lists=nr.debug_list_counts().cpu().numpy())
+    a, b = out[-1], out[1]
+    assert (b["lists"] >= 2).sum() > 1000, "the wave kernel must see many rays"
+    assert np.array_equal(a["n"], b["n"])
+    assert a["n"].max() > 16, "lists must span several rounds"
+    for slot in range(min(64, int(a["n"].max()))):
+        live = a["n"] > slot
+        assert np.array_equal(a["ids"][slot][live], b["ids"][slot][live]), slot
+        assert np.array_equal(a["Th"][slot][live].view(np.uint32), b["Th"][slot][live].view(np.uint32)), slot
+    assert np.array_equal(a["T"].view(np.uint32), b["T"].view(np.uint32))
+    assert np.array_equal(a["D"].view(np.uint32), b["D"].view(np.uint32))
+    assert np.abs(a["F"] - b["F"]).max() < 1e-6
+    assert np.abs(a["C"] - b["C"]).max() < 1e-6
+    assert np.abs(a["E"] - b["E"]).max() < 1e-5

@@ -1,7 +1,9 @@
 """Forward and backward knobs of the reference config on the GPU against the
 oracle: kernel exponent p (exp_power 1, 2, 3), alpha threshold, background,
 depth output (render_depth, depth normalised by 1 - T), and the znear/zfar
-interval, in the collected-lists and the re-traversal forward."""
+interval, in the collected-lists forward (per-lane compositing, and "wave":
+every list of two hits or more composited by a whole wave) and the
+re-traversal forward."""
 import numpy as np
 import pytest
 import torch
@@ -21,14 +23,15 @@ CASES = [
 ]
 
 
-@pytest.mark.parametrize("lists", [None, 0], ids=["lists", "traverse"])
+@pytest.mark.parametrize("lists,wave", [(None, -1), (0, 0), (None, 1)], ids=["lists", "traverse", "wave"])
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_knobs_forward_backward(case, lists):
+def test_knobs_forward_backward(case, lists, wave):
     kw = dict(CASES[case])
     znear, zfar = kw.pop("znear", 0.0), kw.pop("zfar", 99999.9)
     scene = scenes.tiny_scene(1024, seed=20 + case)
     cam = scenes.tiny_camera(40, 32)
-    nr = setup_native(scene, cam, t_threshold=0.02, znear=znear, zfar=zfar, list_capacity=lists, **kw)
+    nr = setup_native(scene, cam, t_threshold=0.02, znear=znear, zfar=zfar, list_capacity=lists,
+                      wave_list_min=wave, **kw)
     nr.get_config().render_depth.fill_(True)
     torch.set_grad_enabled(True)
     nr.forward_pass()
@@ -64,9 +67,10 @@ def test_knobs_forward_backward(case, lists):
         assert rel_err(getattr(g, name).cpu(), ob[key]) < 1e-3, name
 
 
-@pytest.mark.parametrize("lists,hit_cap", [(None, None), (0, None), (3, None), (None, 3)],
-                         ids=["lists", "traverse", "mixed", "recompute-backward"])
-def test_fp16_hits_forward_backward(lists, hit_cap):
+@pytest.mark.parametrize("lists,hit_cap,wave", [(None, None, -1), (0, None, 0), (3, None, 0), (None, 3, -1),
+                                                (None, None, 1), (None, 3, 1)],
+                         ids=["lists", "traverse", "mixed", "recompute-backward", "wave", "wave-recompute-backward"])
+def test_fp16_hits_forward_backward(lists, hit_cap, wave):
     """The reference's fp16 hit storage: distances and compositing alphas
     rounded to fp16, full transmittance from fp32 alphas."""
     scene = scenes.tiny_scene(1024, seed=31)
@@ -74,7 +78,8 @@ def test_fp16_hits_forward_backward(lists, hit_cap):
     bg = (0.2, 0.3, 0.1)
     imgs = {}
     for fp16 in (False, True):
-        nr = setup_native(scene, cam, t_threshold=0.02, bg=bg, list_capacity=lists, hit_capacity=hit_cap)
+        nr = setup_native(scene, cam, t_threshold=0.02, bg=bg, list_capacity=lists, hit_capacity=hit_cap,
+                          wave_list_min=wave)
         nr.fp16_hits = fp16
         nr.get_config().render_depth.fill_(True)
         torch.set_grad_enabled(True)
