@@ -364,15 +364,12 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 #ifndef GRAY_FINAL_STACK
 #define GRAY_FINAL_STACK 64
 #endif
-// Nodes visited and leaves tested per lane and iteration.
-#ifndef GRAY_COLLECT_HEADS
-#define GRAY_COLLECT_HEADS 1
-#endif
+// Leaves tested per lane and iteration.
 #ifndef GRAY_COLLECT_LEAVES
 #define GRAY_COLLECT_LEAVES 1
 #endif
-// Leaf queue per lane (LDS ring, a power of two >= 2 * heads): leaves found by
-// node visits wait here for their exact test.
+// Leaf queue per lane (LDS ring, a power of two >= 4): leaves found by node
+// visits wait here for their exact test.
 #ifndef GRAY_LEAF_QUEUE
 #define GRAY_LEAF_QUEUE 8
 #endif
@@ -389,13 +386,12 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 #endif
 constexpr int kCollectStack = GRAY_COLLECT_STACK;
 constexpr int kFinalStack = GRAY_FINAL_STACK;
-constexpr int kCollectHeads = GRAY_COLLECT_HEADS;
 constexpr int kCollectLeaves = GRAY_COLLECT_LEAVES;
 constexpr int kLeafQueue = GRAY_LEAF_QUEUE;
 constexpr int kTaskRounds = GRAY_TASK_ROUNDS;
 constexpr int kTaskWaves = 4096;           // persistent waves of a task round
 constexpr int32_t kRayDead = -(1 << 30);   // list_n of a ray whose list or tasks overflowed
-static_assert((kLeafQueue & (kLeafQueue - 1)) == 0 && kLeafQueue >= 2 * kCollectHeads, "leaf queue size");
+static_assert((kLeafQueue & (kLeafQueue - 1)) == 0, "leaf queue size: a power of two");
 
 // fp16_hits: the reference keeps each hit's distance and alpha as fp16
 // (per_pixel_linked_list.h:28-29, __float2half_rn) and orders, cuts and
@@ -435,15 +431,16 @@ __device__ __forceinline__ void collect_leaf(const TraceArgs &A, const GeoV &q, 
     }
 }
 
-// One lane's walk of the LBVH for its ray (the order of the walk does not
-// matter here: the list is composited in distance order afterwards), from a
-// node or leaf reference.  One iteration: the exact test of up to
-// kCollectLeaves queued leaves, whose records were requested in the previous
-// iteration, and the box tests of the children of up to kCollectHeads nodes;
-// hit leaves join the queue, hit internal children become the next nodes or
-// go onto the stack.  Queuing keeps the expensive leaf test dense across the
-// wave (a node visit finds a leaf only about one time in four).  Node visits
-// pause while the queue could not take every child.
+// One lane's walk of the 4-wide tree for its ray (the order of the walk does
+// not matter here: the list is composited in distance order afterwards), from
+// a node or leaf reference (binary node indices of even depth; the root is 0).
+// One iteration: the exact test of up to kCollectLeaves queued leaves, whose
+// records were requested in the previous iteration, and the visit of one
+// node, which tests its four child boxes; hit leaves join the queue, the
+// first hit internal child is the next node and the others go onto the stack
+// (or, when it is full, become tasks).  Queuing keeps the expensive leaf test
+// dense across the wave.  Node visits pause while the queue could not take
+// four leaves.
 //
 // Every hit goes to the ray's list: at slot cnt (the per-tile pass owns the
 // ray) or at a slot taken from list_n[g] with an atomic (tasks of one ray run
@@ -454,151 +451,6 @@ __device__ __forceinline__ void collect_leaf(const TraceArgs &A, const GeoV &q, 
 // is then re-traversed instead.  Must be called by the whole wave.
 template <bool kTask, int kStack>
 __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord *__restrict__ geo,
-                                             const BvhNode *__restrict__ nodes, const LaneRay &r, size_t g,
-                                             uint32_t start, int budget, uint32_t *__restrict__ s_stack,
-                                             uint32_t *__restrict__ s_leaf, int lane, int &cnt, bool &ok,
-                                             int &visits, int &leaves) {
-    constexpr int NH = kCollectHeads, NL = kCollectLeaves;
-    const int cap = A.list_cap;
-    const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power;
-    // heads fill from index 0 (h[k] holds a node only when h[k-1] does), and
-    // so do the pending leaves; a pending slot is empty only when the queue is
-    uint32_t h[NH];
-    NodeV nd[NH];
-    uint32_t pl[NL];
-    GeoV pg[NL];
-#pragma unroll
-    for (int k = 0; k < NH; ++k) h[k] = GRAY_EMPTY;
-#pragma unroll
-    for (int l = 0; l < NL; ++l) pl[l] = GRAY_EMPTY;
-    if (start != GRAY_EMPTY) {
-        if (start & GRAY_LEAF) {
-            pl[0] = start & ~GRAY_LEAF;
-            pg[0] = load_geo(geo, pl[0]);
-        } else {
-            h[0] = start;
-            nd[0] = load_node(nodes, start);
-        }
-    }
-    int sp = 0, qh = 0, qn = 0;
-    for (int it = 0;; ++it) {
-        if (!__ballot(h[0] != GRAY_EMPTY || pl[0] != GRAY_EMPTY)) break;
-        if (it == budget) {
-            // hand the remaining work on: heads, stack, pending and queued leaves
-            int m = sp + qn;
-#pragma unroll
-            for (int k = 0; k < NH; ++k) m += h[k] != GRAY_EMPTY;
-#pragma unroll
-            for (int l = 0; l < NL; ++l) m += pl[l] != GRAY_EMPTY;
-            if (m > 0) {
-                // entries past the buffer's end are dropped and the ray is
-                // re-traversed; the ones that fit are still written, so every
-                // entry below the (clamped) count is this launch's
-                const int base = atomicAdd(A.task_count_out, m);
-                if (base + m > A.task_cap) ok = false;
-                const int end = min(base + m, A.task_cap);
-                int o = base;
-                auto put = [&](uint32_t ref) {
-                    if (o < end) A.task_out[o] = make_uint2((uint32_t)g, ref);
-                    ++o;
-                };
-#pragma unroll
-                for (int k = 0; k < NH; ++k)
-                    if (h[k] != GRAY_EMPTY) put(h[k]);
-#pragma unroll
-                for (int l = 0; l < NL; ++l)
-                    if (pl[l] != GRAY_EMPTY) put(pl[l] | GRAY_LEAF);
-                for (int k = 0; k < sp; ++k) put(s_stack[k * GRAY_WAVE + lane]);
-                for (int k = 0; k < qn; ++k) put(s_leaf[((qh + k) & (kLeafQueue - 1)) * GRAY_WAVE + lane] | GRAY_LEAF);
-            }
-            break;
-        }
-#pragma unroll
-        for (int l = 0; l < NL; ++l) {
-            if (pl[l] == GRAY_EMPTY) continue;
-            ++leaves;
-            collect_leaf<kTask>(A, pg[l], pl[l], r, g, cap, athr, p, cnt, ok);
-        }
-        uint32_t nh[NH];
-#pragma unroll
-        for (int k = 0; k < NH; ++k) nh[k] = h[k];
-        bool moved = false;
-        if (h[0] != GRAY_EMPTY && qn <= kLeafQueue - 2 * NH) {
-            moved = true;
-            uint32_t ch[2 * NH];
-#pragma unroll
-            for (int k = 0; k < NH; ++k) {
-                ch[2 * k] = ch[2 * k + 1] = GRAY_EMPTY;
-                if (h[k] == GRAY_EMPTY) continue;
-                ++visits;
-                if (nd[k].left != GRAY_EMPTY && slab_test(r, nd[k].lb, 0.0f)) ch[2 * k] = nd[k].left;
-                if (nd[k].right != GRAY_EMPTY && slab_test(r, nd[k].rb, 0.0f)) ch[2 * k + 1] = nd[k].right;
-            }
-            int nfill = 0;
-#pragma unroll
-            for (int k = 0; k < NH; ++k) nh[k] = GRAY_EMPTY;
-#pragma unroll
-            for (int k = 0; k < 2 * NH; ++k) {
-                const uint32_t c = ch[k];
-                if (c == GRAY_EMPTY) continue;
-                if (c & GRAY_LEAF) {
-                    s_leaf[((qh + qn) & (kLeafQueue - 1)) * GRAY_WAVE + lane] = c & ~GRAY_LEAF;
-                    ++qn;
-                } else if (nfill < NH) {
-#pragma unroll
-                    for (int m = 0; m < NH; ++m) nh[m] = m == nfill ? c : nh[m];
-                    ++nfill;
-                } else if (sp < kStack) {
-                    s_stack[sp * GRAY_WAVE + lane] = c;
-                    ++sp;
-                } else if (budget >= 0) {
-                    const int o = atomicAdd(A.task_count_out, 1);
-                    if (o < A.task_cap) A.task_out[o] = make_uint2((uint32_t)g, c);
-                    else ok = false;
-                } else {
-                    ok = false;
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < NH; ++m) {
-                if (m >= nfill && sp > 0) {
-                    --sp;
-                    nh[m] = s_stack[sp * GRAY_WAVE + lane];
-                }
-            }
-        }
-        if (!ok) {
-#pragma unroll
-            for (int k = 0; k < NH; ++k) nh[k] = GRAY_EMPTY;
-            sp = qn = 0;
-        }
-        // requests: the next leaf records first, then the next nodes
-#pragma unroll
-        for (int l = 0; l < NL; ++l) {
-            pl[l] = GRAY_EMPTY;
-            if (qn > 0) {
-                pl[l] = s_leaf[qh * GRAY_WAVE + lane];
-                qh = (qh + 1) & (kLeafQueue - 1);
-                --qn;
-                pg[l] = load_geo(geo, pl[l]);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < NH; ++k) {
-            h[k] = nh[k];
-            if (moved && h[k] != GRAY_EMPTY) nd[k] = load_node(nodes, h[k]);
-        }
-    }
-}
-
-// The same walk over the 4-wide nodes, one node per iteration: a visit tests
-// the four child boxes; hit leaves join the queue, the first hit internal
-// child is the next node and the others go onto the stack (or, when it is
-// full, become tasks).  Node visits pause while the queue could not take four
-// leaves.  References are binary node indices (of even depth), so tasks and
-// the root are shared with the binary walk.
-template <bool kTask, int kStack>
-__device__ __forceinline__ void collect_walk_wide(const TraceArgs &A, const GeoRecord *__restrict__ geo,
                                                   const Bvh4Node *__restrict__ wide, const LaneRay &r, size_t g,
                                                   uint32_t start, int budget, uint32_t *__restrict__ s_stack,
                                                   uint32_t *__restrict__ s_leaf, int lane, int &cnt, bool &ok,
@@ -710,29 +562,11 @@ __device__ __forceinline__ void collect_walk_wide(const TraceArgs &A, const GeoR
     }
 }
 
-// Which tree the collect pass walks: the 4-wide collapse (1) or the binary
-// LBVH (0).  Both give the same hit lists; they differ in speed.
-#ifndef GRAY_COLLECT_WIDE
-#define GRAY_COLLECT_WIDE 1
-#endif
-#if GRAY_COLLECT_WIDE
-using CollectNode = Bvh4Node;
 static_assert(kLeafQueue >= 4, "a 4-wide visit may queue four leaves");
-template <bool kTask, int kStack, typename... T>
-__device__ __forceinline__ void collect_any(T &&...a) {
-    collect_walk_wide<kTask, kStack>(static_cast<T &&>(a)...);
-}
-#else
-using CollectNode = BvhNode;
-template <bool kTask, int kStack, typename... T>
-__device__ __forceinline__ void collect_any(T &&...a) {
-    collect_walk<kTask, kStack>(static_cast<T &&>(a)...);
-}
-#endif
 
 // Per-tile collect pass: every ray of the tile walks from the root.
 __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRecord *__restrict__ geo,
-                                                       const CollectNode *__restrict__ nodes) {
+                                                       const Bvh4Node *__restrict__ nodes) {
     __shared__ uint32_t s_stack[kCollectStack * GRAY_WAVE];
     __shared__ uint32_t s_leaf[kLeafQueue * GRAY_WAVE];
     int vi, tx, ty, tile;
@@ -746,7 +580,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
     const size_t g = (size_t)tile * GRAY_WAVE + lane;
     int cnt = 0, visits = 0, leaves = 0;
     bool ok = true;
-    collect_any<false, kCollectStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
+    collect_walk<false, kCollectStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
                                       A.budget, s_stack, s_leaf, lane, cnt, ok, visits, leaves);
     A.list_n[g] = ok ? cnt : -1;
     if (A.tile_stats) {
@@ -786,7 +620,7 @@ __device__ __forceinline__ const DevView &lane_pixel(const TraceArgs &A, uint32_
 // passes it.
 template <int kStack>
 __global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const GeoRecord *__restrict__ geo,
-                                                             const CollectNode *__restrict__ nodes) {
+                                                             const Bvh4Node *__restrict__ nodes) {
     __shared__ uint32_t s_stack[kStack * GRAY_WAVE];
     __shared__ uint32_t s_leaf[kLeafQueue * GRAY_WAVE];
     const int lane = threadIdx.x;
@@ -811,7 +645,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const 
         }
         int cnt = 0, visits = 0, leaves = 0;
         bool ok = true;
-        collect_any<true, kStack>(A, geo, nodes, r, g, start, A.budget, s_stack, s_leaf, lane, cnt, ok, visits,
+        collect_walk<true, kStack>(A, geo, nodes, r, g, start, A.budget, s_stack, s_leaf, lane, cnt, ok, visits,
                                   leaves);
         if (!ok && start != GRAY_EMPTY) atomicExch(A.list_n + g, kRayDead);
     }
@@ -2227,11 +2061,7 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             A.task_count_out = count;
             A.budget = args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
             if ((rc = mark(0))) return rc;
-#if GRAY_COLLECT_WIDE
-            const CollectNode *cnodes = ctx->wide;
-#else
-            const CollectNode *cnodes = ctx->nodes;
-#endif
+            const Bvh4Node *cnodes = ctx->wide;
             hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, cnodes);
             GRAY_HIP(hipGetLastError());
             if ((rc = mark(1))) return rc;
