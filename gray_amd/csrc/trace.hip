@@ -379,10 +379,10 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 #define GRAY_COLLECT_BUDGET 192
 #endif
 #ifndef GRAY_TASK_BUDGET
-#define GRAY_TASK_BUDGET 96
+#define GRAY_TASK_BUDGET 48
 #endif
 #ifndef GRAY_TASK_ROUNDS
-#define GRAY_TASK_ROUNDS 3
+#define GRAY_TASK_ROUNDS 4
 #endif
 constexpr int kCollectStack = GRAY_COLLECT_STACK;
 constexpr int kFinalStack = GRAY_FINAL_STACK;
@@ -477,26 +477,54 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
     for (int it = 0;; ++it) {
         if (!__ballot(h != GRAY_EMPTY || pl[0] != GRAY_EMPTY)) break;
         if (it == budget) {
+            // the wave's remaining work (per lane: stack from the bottom, node,
+            // pending and queued leaves) goes to the task buffer slot-major,
+            // entry k of every lane before entry k + 1 of any, so a wave of
+            // the next round takes the same stack slot of neighbouring rays,
+            // which mostly walk the same subtrees.  Entries past the buffer's
+            // end are dropped and their ray is re-traversed.
             int m = sp + qn + (h != GRAY_EMPTY);
 #pragma unroll
             for (int l = 0; l < NL; ++l) m += pl[l] != GRAY_EMPTY;
-            if (m > 0) {
-                // entries past the buffer's end are dropped and the ray is
-                // re-traversed; the ones that fit are still written
-                const int base = atomicAdd(A.task_count_out, m);
-                if (base + m > A.task_cap) ok = false;
-                const int end = min(base + m, A.task_cap);
-                int o = base;
-                auto put = [&](uint32_t ref) {
-                    if (o < end) A.task_out[o] = make_uint2((uint32_t)g, ref);
-                    ++o;
-                };
-                if (h != GRAY_EMPTY) put(h);
+            int mmax = m, tot = m;
 #pragma unroll
-                for (int l = 0; l < NL; ++l)
-                    if (pl[l] != GRAY_EMPTY) put(pl[l] | GRAY_LEAF);
-                for (int k = 0; k < sp; ++k) put(s_stack[k * GRAY_WAVE + lane]);
-                for (int k = 0; k < qn; ++k) put(s_leaf[((qh + k) & (kLeafQueue - 1)) * GRAY_WAVE + lane] | GRAY_LEAF);
+            for (int off = 32; off > 0; off >>= 1) {
+                mmax = max(mmax, __shfl_xor(mmax, off));
+                tot += __shfl_xor(tot, off);
+            }
+            if (tot > 0) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(A.task_count_out, tot);
+                base = __shfl(base, 0);
+                const uint64_t below = (1ull << lane) - 1ull;
+                int acc = 0;
+                for (int k = 0; k < mmax; ++k) {
+                    const uint64_t has = __ballot(m > k);
+                    if (m > k) {
+                        uint32_t ref = GRAY_EMPTY;
+                        if (k < sp) {
+                            ref = s_stack[k * GRAY_WAVE + lane];
+                        } else {
+                            int j = k - sp;
+                            if (h != GRAY_EMPTY) {
+                                if (j == 0) ref = h;
+                                --j;
+                            }
+#pragma unroll
+                            for (int l = 0; l < NL; ++l) {
+                                if (pl[l] != GRAY_EMPTY) {
+                                    if (j == 0) ref = pl[l] | GRAY_LEAF;
+                                    --j;
+                                }
+                            }
+                            if (ref == GRAY_EMPTY) ref = s_leaf[((qh + j) & (kLeafQueue - 1)) * GRAY_WAVE + lane] | GRAY_LEAF;
+                        }
+                        const int o = base + acc + __popcll(has & below);
+                        if (o < A.task_cap) A.task_out[o] = make_uint2((uint32_t)g, ref);
+                        else ok = false;
+                    }
+                    acc += __popcll(has);
+                }
             }
             break;
         }
