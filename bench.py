@@ -9,6 +9,11 @@ roofline (HIP-event launch time, HBM traffic from rocprofv3 counters) and a
 bounded CPU-oracle baseline timed on the host cores.
 
   python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-traffic]
+
+Other workloads (--workload): render (3M Gaussians, 1600x1063, 120 poses),
+fwdbwd (200K Gaussians, 8 cameras at 512x512 in one launch), serve (2M
+Gaussians, Poisson requests of mixed resolution) and tiny (1,024 Gaussians,
+64x64, against the CPU oracle on the whole frame).
 """
 from __future__ import annotations
 
@@ -488,6 +493,85 @@ def run_fwdbwd(args):
     }
 
 
+def run_tiny(args):
+    """Plain forward + backward (1,024 Gaussians: means N(0, 0.5^2), log-scales
+    N(-3, 0.5^2), SH degree 3; one 64x64 camera at distance 3, fov 60 deg)
+    through the drop-in Raytracer, against the C oracle on the whole frame:
+    image max-abs and gradient relative errors, and the oracle's own time on
+    the host cores (its snapshot, SH, forward and backward) beside the GPU's."""
+    import numpy as np
+    import torch
+
+    from gray_amd import RaytracerConfig, scenes
+    from oracle import oracle as O
+
+    n = args.gaussians or 1024
+    scene = scenes.tiny_scene(n, seed=0)
+    cam = scenes.tiny_camera(64, 64)
+    cfg = RaytracerConfig()
+    rt = setup_raytracer(scene, cam, cfg)
+    W, H = cam.image_width, cam.image_height
+    wgt = torch.randn(3, H, W, generator=torch.Generator().manual_seed(5)).cuda()
+    g = rt.cuda_module.get_gaussians()
+    names = ("grad_mean", "grad_rotation", "grad_scale", "grad_opacity", "grad_channels")
+
+    def fwd_bwd():
+        with torch.no_grad():
+            for name in names:
+                getattr(g, name).zero_()
+        rt.backward((rt(cam) * wgt).sum())
+
+    for _ in range(args.warmup):
+        fwd_bwd()
+    torch.cuda.synchronize()
+    gpu_ms, _ = _median_ms(fwd_bwd, args.steps)
+    rt.cuda_module.device_status()
+    with torch.no_grad():
+        for name in names:
+            getattr(g, name).zero_()
+    render = rt(cam)
+    img = render.detach().permute(1, 2, 0).reshape(-1, 3).cpu().numpy()
+    rt.backward((render * wgt).sum())
+    torch.cuda.synchronize()
+    got = {k: getattr(g, k).cpu().numpy().copy() for k in names}
+
+    # the C oracle: the same frame from the same parameters, timed on the host
+    origin = np.asarray(cam.origin, np.float32)
+    ro, rd = O.pinhole_rays(W, H, cam.fov_y, origin, cam.rotation_c2w_blender())
+    k = O.make_knobs(exp_power=cfg.exp_power, alpha_threshold=cfg.alpha_threshold, t_threshold=cfg.t_threshold)
+    gout = wgt.permute(1, 2, 0).reshape(-1, 3).cpu().numpy().astype(np.float32)
+    t0 = time.perf_counter()
+    geo, l2w = O.snapshot(scene.mean.numpy(), scene.rotation.numpy(), scene.scale.numpy(), scene.opacity.numpy(),
+                          cfg.alpha_threshold, cfg.exp_power)
+    rgb = O.sh_forward(scene.mean.numpy(), scene.sh_dc.numpy(), scene.sh_rest.numpy(), origin, 3)
+    ref = O.render(geo, rgb, ro, rd, k, cap=256)
+    ob = O.backward(geo, l2w, rgb, scene.rotation.numpy(), scene.scale.numpy(), scene.opacity.numpy(), ro, rd, ref,
+                    gout, k, float(W * H))
+    cpu_ms = (time.perf_counter() - t0) * 1e3
+
+    def rel(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+    grad_err = {key: rel(got[name].reshape(-1), ob[key].reshape(-1))
+                for name, key in (("grad_opacity", "opacity"), ("grad_scale", "scale"),
+                                  ("grad_rotation", "rotation"), ("grad_channels", "rgb"))}
+    alg = 3 * 236 * n + 24 * W * H
+    return {
+        "metric": "forward+backward ms, 1,024 Gaussians 64x64 (image and gradients against the CPU oracle)",
+        "value": gpu_ms, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": gpu_ms, "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded random Gaussians)",
+        "config": {"workload": "fwdbwd_1K_64x64", "gaussians": n, "width": W, "height": H},
+        "image_max_abs_vs_oracle": float(np.abs(img - ref["C"]).max()),
+        "grad_rel_err_vs_oracle": grad_err,
+        "roofline": {"bound": "hbm", "kernel": "fwd+bwd", "achieved": round(alg / (gpu_ms * 1e-3) / 1e9, 3),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(alg / (gpu_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 8), "traffic": None},
+        "cpu_baseline": {"value": cpu_ms, "unit": "ms", "cores": O.num_threads(), "kind": "port",
+                         "sample": f"the whole {W}x{H} frame: snapshot, SH, forward and backward (C oracle)"},
+    }
+
+
 def run_serve(args):
     """Serving: 2M Gaussians resident (60 surfaces, log-scale N(-5, 0.3^2)),
     Poisson arrivals of camera requests (sphere of radius 4-8, resolutions
@@ -542,7 +626,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="train", choices=["train", "render", "fwdbwd", "serve"])
+    ap.add_argument("--workload", default="train", choices=["train", "render", "fwdbwd", "serve", "tiny"])
     ap.add_argument("--serve-requests", type=int, default=120)
     ap.add_argument("--serve-slots", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=float, default=48.0)
@@ -563,7 +647,7 @@ def main():
         import torch
 
         torch.cuda.set_device(0)
-        out = {"render": run_render, "fwdbwd": run_fwdbwd, "serve": run_serve}[args.workload](args)
+        out = {"render": run_render, "fwdbwd": run_fwdbwd, "serve": run_serve, "tiny": run_tiny}[args.workload](args)
         print(json.dumps(out), flush=True)
         return
     if args.gaussians is None:
