@@ -14,7 +14,13 @@ hit-list buffers, reserved up front) and its own stream.  The host loop:
     on the slot's stream and the loop goes on, so batches of different slots
     run on the device together while new requests keep arriving.
 Plain mode is the comparison: one request per launch, in arrival order, on a
-single slot.
+single slot.  `submit`/`poll` drive the same admission without a clock of
+their own, for request sources that arrive over time (the remote viewer).
+
+A request's display mode is the reference viewer's (view.py:163-190):
+"Gaussians" (the image), "Depth" (the image and the depth output) or
+"Ellipsoids" (the ellipsoid display, its own launch).  Requests of one batch
+share the launch kind, and ellipsoid batches their minimum opacity.
 
 The per-request work is the reference's forward_pass_display
 (cuda/raytracer.h:104-114) for that camera.  Only the request's own view
@@ -39,7 +45,8 @@ F32 = torch.float32
 RESOLUTIONS = ((256, 256), (512, 384), (1280, 720), (1600, 900))
 MAX_VIEWS = 8
 TASK_BYTES_PER_LANE = 256
-CAM_FLOATS = 16  # origin 3, c2w 9, fov, znear, zfar, pad
+CAM_FLOATS = 16  # origin 3, c2w 9, fov, znear, zfar, ellipsoid minimum opacity
+MODES = ("Gaussians", "Depth", "Ellipsoids")
 
 
 def lanes_of(width: int, height: int) -> int:
@@ -54,6 +61,16 @@ class Request:
     done: float = math.nan        # device completion, seconds after the start
     batch_size: int = 0
     image: Optional[torch.Tensor] = field(default=None, repr=False)
+    mode: str = "Gaussians"
+    znear: Optional[float] = None     # None: the server's setting
+    min_opacity: float = 0.05         # ellipsoid display
+    depth: Optional[torch.Tensor] = field(default=None, repr=False)  # [h, w, 1] in "Depth" mode
+    error: Optional[str] = None       # set by poll() for a request it could not admit
+    tag: object = field(default=None, repr=False)  # the caller's
+
+    def __post_init__(self):
+        if self.mode not in MODES:
+            raise ValueError(f"unknown display mode {self.mode!r} (one of {', '.join(MODES)})")
 
     @property
     def width(self) -> int:
@@ -102,6 +119,8 @@ class SceneServer:
                    t(s.eps_forward_normalization), t(s.eps_scale_grad),
                    torch.tensor(list(s.background), dtype=F32, device=self.device), t(False, torch.bool),
                    t(True, torch.bool)]
+        self._k_depth = list(self._k)
+        self._k_depth[7] = t(True, torch.bool)
         self._deg = torch.tensor([s.sh_degree], dtype=torch.int32, device=self.device)
         p = L.Params()
         p.count, p.num_sh_coeffs, p.sh_is_fp16 = self.n, self.tensors[5].shape[1], 0
@@ -112,9 +131,26 @@ class SceneServer:
         self.slot_lanes = int(budget_bytes // max(1, slots) // per_lane)
         self.budget_bytes = budget_bytes
         self.slots = [_Slot(self, self.slot_lanes) for _ in range(slots)]
+        self._queue: List[Request] = []
+        self._h0 = time.perf_counter()
 
-    def knobs(self) -> L.Knobs:
-        return L.Knobs(*(k.data_ptr() for k in self._k))
+    def knobs(self, depth: bool = False) -> L.Knobs:
+        return L.Knobs(*(k.data_ptr() for k in (self._k_depth if depth else self._k)))
+
+    def set_global_scale_factor(self, value: float) -> None:
+        """The viewer's scaling modifier (view.py:171-176, 403-406): waits for
+        the batches in flight (they keep their images, made with the old
+        factor; poll() still returns them) and rebuilds every slot's BVH."""
+        if float(self._k[3].item()) == float(value):
+            return
+        for sl in self.slots:
+            sl.stream.synchronize()
+        self._k[3].fill_(float(value))
+        self._k_depth[3].fill_(float(value))
+        torch.cuda.current_stream(self.device).synchronize()
+        for sl in self.slots:
+            sp = L.stream_ptr(sl.stream)
+            L.check(L.LIB.gray_bvh_update(sl.ctx.handle, C.byref(self.params), C.byref(self.knobs()), 1, sp))
 
     # ---- one batch ---------------------------------------------------------
     def _launch(self, slot: _Slot, reqs: Sequence[Request]) -> None:
@@ -125,15 +161,21 @@ class SceneServer:
             row = np.zeros(CAM_FLOATS, np.float32)
             row[0:3] = c.origin
             row[3:12] = c.rotation_c2w_blender().reshape(-1)
-            row[12], row[13], row[14] = c.fov_y, s.znear, s.zfar
+            row[12], row[13], row[14] = c.fov_y, s.znear if r.znear is None else r.znear, s.zfar
+            row[15] = r.min_opacity
             slot.cams_host[i] = torch.from_numpy(row)
         views = (L.View * V)()
         outs = []
         with torch.cuda.stream(slot.stream):
             slot.cams[:V].copy_(slot.cams_host[:V], non_blocking=True)
+            ellipsoids = reqs[0].mode == "Ellipsoids"
+            depth_on = any(r.mode == "Depth" for r in reqs)
             for i, r in enumerate(reqs):
                 img = torch.empty(r.height, r.width, 3, dtype=F32, device=self.device)
                 outs.append(img)
+                r.depth = None
+                if r.mode == "Depth":
+                    r.depth = torch.zeros(r.height, r.width, 1, dtype=F32, device=self.device)
                 v = views[i]
                 base = slot.cams[i]
                 v.origin = base.data_ptr()
@@ -144,10 +186,11 @@ class SceneServer:
                 v.width, v.height, v.stride, v.alloc_height = r.width, r.height, r.width, r.height
                 v.rgb = slot.rgb[i].data_ptr()
                 v.out_channels = img.data_ptr()
+                v.out_depth = r.depth.data_ptr() if r.depth is not None else None
             sp = L.stream_ptr(slot.stream)
             origins = (C.c_void_p * V)(*[views[i].origin for i in range(V)])
             rgbs = (C.c_void_p * V)(*[views[i].rgb for i in range(V)])
-            knobs = self.knobs()
+            knobs = self.knobs(depth_on)
             L.check(L.LIB.gray_sh_forward(slot.ctx.handle, C.byref(self.params), C.byref(knobs), V, origins, rgbs, sp))
             a = L.RenderArgs()
             a.knobs = knobs
@@ -155,7 +198,11 @@ class SceneServer:
             a.views = C.cast(views, C.POINTER(L.View))
             a.save, a.hit_capacity, a.use_bvh, a.list_capacity = 0, 0, 1, self.list_capacity
             a.fp16_hits = 1 if s.fp16_hits else 0
-            L.check(L.LIB.gray_render_forward(slot.ctx.handle, C.byref(a), sp))
+            if ellipsoids:
+                min_op = C.c_void_p(slot.cams[0].data_ptr() + 15 * 4)
+                L.check(L.LIB.gray_render_ellipsoids(slot.ctx.handle, C.byref(a), min_op, sp))
+            else:
+                L.check(L.LIB.gray_render_forward(slot.ctx.handle, C.byref(a), sp))
             slot.end = torch.cuda.Event(enable_timing=True)
             slot.end.record(slot.stream)
         for r, img in zip(reqs, outs):
@@ -180,14 +227,56 @@ class SceneServer:
                              f"budget of {slot.max_lanes} lanes")
         if not batched:
             return [queue.pop(0)]
+        head = queue[0]
+        kind = lambda r: (r.mode == "Ellipsoids", r.min_opacity if r.mode == "Ellipsoids" else 0.0)  # noqa: E731
         out, lanes = [], 0
-        while queue and len(out) < MAX_VIEWS:
+        while queue and len(out) < MAX_VIEWS and kind(queue[0]) == kind(head):
             need = lanes_of(queue[0].width, queue[0].height)
             if lanes + need > slot.max_lanes:
                 break
             lanes += need
             out.append(queue.pop(0))
         return out
+
+    def submit(self, request: Request) -> None:
+        """Queues a request for the next poll(); its arrival is stamped now
+        (seconds since the server was made)."""
+        request.arrival = time.perf_counter() - self._h0
+        request.done = math.nan
+        self._queue.append(request)
+
+    def poll(self) -> List[Request]:
+        """Retires the slots whose batch finished (requests get .done in the
+        same clock as .arrival) and admits queued requests into free slots.
+        Returns the finished requests, and those that could not be admitted
+        (with .error set).  Does not block."""
+        finished: List[Request] = []
+        for sl in self.slots:
+            if sl.batch and sl.end.query():
+                L.check(L.LIB.gray_device_status(sl.ctx.handle, L.stream_ptr(sl.stream)))
+                t = time.perf_counter() - self._h0
+                for r in sl.batch:
+                    r.done = t
+                finished.extend(sl.batch)
+                sl.batch = []
+        for sl in self.slots:
+            if not self._queue:
+                break
+            if sl.batch:
+                continue
+            try:
+                reqs = self._take(self._queue, sl, True)
+            except ValueError as e:
+                r = self._queue.pop(0)
+                r.error = str(e)
+                finished.append(r)
+                continue
+            self._launch(sl, reqs)
+        return finished
+
+    @property
+    def busy(self) -> bool:
+        return bool(self._queue) or any(sl.batch for sl in self.slots)
 
     def serve(self, requests: Sequence[Request], batched: bool = True) -> dict:
         """Runs the arrival process in real time; returns latency statistics

@@ -69,3 +69,84 @@ def test_arrivals_over_time(served):
     stats = server.serve(reqs, batched=True)
     assert all(not math.isnan(r.done) and r.done >= r.arrival for r in reqs)
     assert stats["frames_per_s"] > 0
+
+
+def _drop_in_display(scene, cam, **fills):
+    """The drop-in path's forward_pass_display with config knobs set."""
+    nr = setup_native(scene, cam, t_threshold=0.03)
+    cfg = nr.get_config()
+    for k, v in fills.items():
+        getattr(cfg, k).fill_(v)
+    nr.rebuild_bvh()
+    with torch.no_grad():
+        nr.forward_pass_display()
+    torch.cuda.synchronize()
+    nr.device_status()
+    fb = nr.get_framebuffer()
+    H, W = cam.image_height, cam.image_width
+    return fb.output_channels[:H, :W].clone(), fb.output_depth[:H, :W].clone()
+
+
+def test_display_modes_match_drop_in(served):
+    """Depth and ellipsoid requests (the viewer's display modes) against the
+    drop-in path with config.render_depth / render_ellipsoids."""
+    scene, server = served
+    cam = small_requests(1, seed=5)[0].camera
+    ref, ref_depth = _drop_in_display(scene, cam, render_depth=True)
+    r = Request(0, cam, mode="Depth")
+    img = server.render_alone(r)
+    assert (ref_depth > 0).sum().item() > 100
+    assert (img - ref).abs().max().item() <= 1e-6
+    assert (r.depth - ref_depth).abs().max().item() <= 1e-5
+    ref, _ = _drop_in_display(scene, cam, render_ellipsoids=True, ellipsoid_min_opacity=0.3)
+    img = server.render_alone(Request(1, cam, mode="Ellipsoids", min_opacity=0.3))
+    assert (img - ref).abs().max().item() <= 1e-6
+    assert (ref.sum(-1) > 0).sum().item() > 100
+    # mixed modes in one serve(): batches split by launch kind, images unchanged
+    reqs = [Request(i, c.camera, mode=("Gaussians", "Ellipsoids", "Depth")[i % 3], min_opacity=0.3)
+            for i, c in enumerate(small_requests(9, seed=6))]
+    alone = [server.render_alone(Request(i, q.camera, mode=q.mode, min_opacity=0.3)).clone()
+             for i, q in enumerate(reqs)]
+    server.serve(reqs, batched=True)
+    for q, a in zip(reqs, alone):
+        assert (q.image - a).abs().max().item() <= 1e-6, (q.rid, q.mode)
+
+
+def test_scale_factor_rebuilds_every_slot(served):
+    scene, server = served
+    cam = small_requests(1, seed=7)[0].camera
+    base = server.render_alone(Request(0, cam)).clone()
+    ref, _ = _drop_in_display(scene, cam, global_scale_factor=1.5)
+    server.set_global_scale_factor(1.5)
+    try:
+        for sl in range(len(server.slots)):
+            r = Request(sl, cam)
+            server._launch(server.slots[sl], [r])
+            server.slots[sl].stream.synchronize()
+            server.slots[sl].batch = []
+            assert (r.image - ref).abs().max().item() <= 1e-6
+        assert (ref - base).abs().max().item() > 1e-3
+    finally:
+        server.set_global_scale_factor(1.0)
+    assert (server.render_alone(Request(0, cam)) - base).abs().max().item() <= 1e-6
+
+
+def test_submit_poll(served):
+    _, server = served
+    reqs = small_requests(10, seed=9)
+    alone = [server.render_alone(r).clone() for r in small_requests(10, seed=9)]
+    for r in reqs:
+        server.submit(r)
+    done = []
+    for _ in range(200000):
+        done += server.poll()
+        if len(done) == len(reqs):
+            break
+    assert not server.busy and sorted(r.rid for r in done) == list(range(10))
+    for r, a in zip(reqs, alone):
+        assert r.error is None and r.done >= r.arrival
+        assert (r.image - a).abs().max().item() <= 1e-6
+    big = Request(99, scenes.sphere_camera(np.random.default_rng(0), 2.5, 4.0, 4000, 4000))
+    server.submit(big)
+    out = server.poll()
+    assert out == [big] and "exceeds" in big.error
