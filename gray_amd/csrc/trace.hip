@@ -402,9 +402,14 @@ static_assert((kLeafQueue & (kLeafQueue - 1)) == 0, "leaf queue size: a power of
 // transmittance.
 __device__ __forceinline__ float fp16_round(float x, bool on) { return on ? __half2float(__float2half_rn(x)) : x; }
 
+// Hit records are written once and read back only by the compositing pass:
+// a non-temporal store (a wave writes 1 KB of whole lines per slot) keeps them
+// from evicting the nodes and leaf records that the walk re-reads.
+typedef float f4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void put_hit(const TraceArgs &A, size_t g, int slot, float dist, uint32_t gid, float a) {
     const size_t e = (size_t)slot * (size_t)A.list_stride + g;
-    A.list_rec[e] = make_float4(dist, __uint_as_float(gid), a, 0.0f);
+    const f4v rec = {dist, __uint_as_float(gid), a, 0.0f};
+    __builtin_nontemporal_store(rec, reinterpret_cast<f4v *>(A.list_rec + e));
 }
 
 // Exact test of leaf gid for the lane's ray; a hit goes to the ray's list at
