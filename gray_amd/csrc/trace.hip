@@ -1619,17 +1619,56 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored(TraceArgs A
     const int n = (x < V.width && y < V.height) ? (int)V.n_comp[pix] : 0;
     LaneRay r;
     PixelState ps;
-    if (n > 0 && n <= A.capacity && load_pixel_state(A, V, pix, r, ps)) {
-        const float p = *A.knobs.exp_power, eps_sg = *A.knobs.eps_scale_grad;
-        const int R = V.ray_count;
-        float gT = 0.0f;
-        auto sink = [&](uint32_t gid, const float *t) { tab.add(A, V, gid, t); };
-        float Tafter = ps.Tf;  // fp16_hits: transmittances rebuilt from the final one
-        for (int k = n - 1; k >= 0; --k) {
-            const uint32_t gid = V.hit_ids[(size_t)k * R + pix];
+    // Every lane runs the wave's longest list (iteration it takes entry
+    // m - 1 - it, back to front) so that the DPP exchanges below see all 64
+    // lanes; lanes past their own list carry no hit.
+    const bool live = n > 0 && n <= A.capacity && load_pixel_state(A, V, pix, r, ps);
+    const int m = live ? n : 0;
+    int mmax = m;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mmax = max(mmax, __shfl_xor(mmax, off));
+    const float p = *A.knobs.exp_power, eps_sg = *A.knobs.eps_scale_grad;
+    const int R = V.ray_count;
+    float gT = 0.0f;
+    float Tafter = ps.Tf;  // fp16_hits: transmittances rebuilt from the final one
+    for (int it = 0; it < mmax; ++it) {
+        const int k = m - 1 - it;
+        float t[kGradTerms];
+#pragma unroll
+        for (int j = 0; j < kGradTerms; ++j) t[j] = 0.0f;
+        uint32_t gid = GRAY_EMPTY;
+        if (k >= 0) {
+            const uint32_t g = V.hit_ids[(size_t)k * R + pix];
             const float Ti = V.hit_T[(size_t)k * R + pix];
-            backward_hit(A, V, r, p, eps_sg, ps, gid, Ti, gT, sink, A.fp16 != 0, Tafter);
+            auto keep = [&](uint32_t gg, const float *tt) {
+#pragma unroll
+                for (int j = 0; j < kGradTerms; ++j) t[j] = tt[j];
+                gid = gg;
+            };
+            backward_hit(A, V, r, p, eps_sg, ps, g, Ti, gT, keep, A.fp16 != 0, Tafter);
         }
+        // Neighbouring pixels often composite the same Gaussian at the same
+        // step: combining their terms first saves LDS table operations, which
+        // bound this kernel.  Horizontal neighbours (lane ^ 1) holding the
+        // same Gaussian: the even one adds both.
+        const uint32_t g1 = dpp_u32<0xB1, 0xF>(gid);
+        const bool pair1 = gid != GRAY_EMPTY && g1 == gid;
+#pragma unroll
+        for (int j = 0; j < kGradTerms; ++j) {
+            const float u = dpp_f32<0xB1, 0xF>(t[j]);
+            if (pair1) t[j] += u;
+        }
+        if (pair1 && (lane & 1)) gid = GRAY_EMPTY;
+        // then vertical neighbours (lane ^ 8, same 16-lane row): the upper one adds both
+        const uint32_t g2 = dpp_u32<0x128, 0xF>(gid);
+        const bool pair2 = gid != GRAY_EMPTY && g2 == gid;
+#pragma unroll
+        for (int j = 0; j < kGradTerms; ++j) {
+            const float u = dpp_f32<0x128, 0xF>(t[j]);
+            if (pair2) t[j] += u;
+        }
+        if (pair2 && (lane & 8)) gid = GRAY_EMPTY;
+        if (gid != GRAY_EMPTY) tab.add(A, V, gid, t);
     }
     __syncthreads();
     tab.flush(A, V, lane);
