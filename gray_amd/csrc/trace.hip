@@ -1659,6 +1659,15 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored(TraceArgs A
             if (pair1) t[j] += u;
         }
         if (pair1 && (lane & 1)) gid = GRAY_EMPTY;
+        // then pixels two apart in the row (lane ^ 2): the left one adds both
+        const uint32_t g3 = dpp_u32<0x4E, 0xF>(gid);
+        const bool pair3 = gid != GRAY_EMPTY && g3 == gid;
+#pragma unroll
+        for (int j = 0; j < kGradTerms; ++j) {
+            const float u = dpp_f32<0x4E, 0xF>(t[j]);
+            if (pair3) t[j] += u;
+        }
+        if (pair3 && (lane & 2)) gid = GRAY_EMPTY;
         // then vertical neighbours (lane ^ 8, same 16-lane row): the upper one adds both
         const uint32_t g2 = dpp_u32<0x128, 0xF>(gid);
         const bool pair2 = gid != GRAY_EMPTY && g2 == gid;
