@@ -29,6 +29,16 @@ i32 = C.c_int32
 i64 = C.c_int64
 
 
+class FrameArgs(C.Structure):
+    _fields_ = [
+        ("origin", vp), ("rotation_c2w", vp), ("rotation_w2c", vp), ("vertical_fov", vp), ("znear", vp),
+        ("zfar", vp), ("rays_from_python", vp), ("grads_enabled", vp), ("total_num_calls", vp),
+        ("was_visible", vp), ("count", i64), ("origin_value", C.c_float * 3), ("c2w_value", C.c_float * 9),
+        ("fov_value", C.c_float), ("znear_value", C.c_float), ("zfar_value", C.c_float),
+        ("grads_enabled_value", i32),
+    ]
+
+
 class Knobs(C.Structure):
     _fields_ = [
         ("exp_power", vp), ("alpha_threshold", vp), ("t_threshold", vp), ("global_scale_factor", vp),
@@ -105,6 +115,7 @@ def _load():
         "gray_last_error": (C.c_char_p, []),
         "gray_abi_version": (i32, []),
         "gray_device_status": (i32, [vp, vp]),
+        "gray_frame_setup": (i32, [C.POINTER(FrameArgs), vp]),
         "gray_bvh_update": (i32, [vp, C.POINTER(Params), C.POINTER(Knobs), i32, vp]),
         "gray_bvh_set_timing": (i32, [vp, i32]),
         "gray_bvh_last_build_ms": (C.c_float, [vp]),
@@ -134,7 +145,7 @@ def _load():
     return lib
 
 
-ABI_VERSION = 8  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
+ABI_VERSION = 9  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
 LIB = _load()
 if LIB.gray_abi_version() != ABI_VERSION:
     raise ImportError(f"libgray_hip.so has ABI {LIB.gray_abi_version()}, the binding expects {ABI_VERSION}; "
@@ -145,7 +156,7 @@ EXPORTED = (
     "gray_sh_backward", "gray_render_forward", "gray_render_backward", "gray_adam_step", "gray_debug_hitlists",
     "gray_debug_snapshot", "gray_debug_bvh", "gray_debug_list_counts", "gray_render_stage_ms",
     "gray_render_reserve", "gray_loss_workspace_bytes", "gray_loss_forward", "gray_loss_backward",
-    "gray_render_ellipsoids", "gray_debug_wide",
+    "gray_render_ellipsoids", "gray_debug_wide", "gray_frame_setup",
 )
 
 

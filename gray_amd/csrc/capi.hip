@@ -68,6 +68,49 @@ extern "C" void gray_ctx_destroy(gray_ctx *ctx) {
     delete ctx;
 }
 
+// One thread writes the camera, flag and counter fields; every thread clears
+// 16 bytes of was_visible (bytewise at an unaligned base or the tail).
+__global__ void k_frame_setup(gray_frame_args a, int aligned16) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        if (a.origin)
+            for (int c = 0; c < 3; ++c) a.origin[c] = a.origin_value[c];
+        if (a.rotation_c2w)
+            for (int c = 0; c < 9; ++c) a.rotation_c2w[c] = a.c2w_value[c];
+        if (a.rotation_w2c)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) a.rotation_w2c[3 * r + c] = a.c2w_value[3 * c + r];
+        if (a.vertical_fov) *a.vertical_fov = a.fov_value;
+        if (a.znear) *a.znear = a.znear_value;
+        if (a.zfar) *a.zfar = a.zfar_value;
+        if (a.rays_from_python) *a.rays_from_python = 0;
+        if (a.grads_enabled) *a.grads_enabled = a.grads_enabled_value != 0 ? 1 : 0;
+        if (a.total_num_calls) *a.total_num_calls += 1;
+    }
+    if (a.was_visible) {
+        const int64_t b = i * 16;
+        if (aligned16 && b + 16 <= a.count) {
+            *reinterpret_cast<uint4 *>(a.was_visible + b) = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            for (int64_t j = b; j < b + 16 && j < a.count; ++j) a.was_visible[j] = 0;
+        }
+    }
+}
+
+extern "C" int gray_frame_setup(const gray_frame_args *args, void *stream) {
+    if (!args || args->count < 0) {
+        gray_set_error(GRAY_ERR_INVALID, "frame_setup: null arguments or negative count");
+        return GRAY_ERR_INVALID;
+    }
+    const int64_t count = args->was_visible ? args->count : 0;
+    const int64_t threads = count > 0 ? (count + 15) / 16 : 1;
+    const int aligned16 = (((uintptr_t)args->was_visible) & 15u) == 0;
+    hipLaunchKernelGGL(k_frame_setup, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       *args, aligned16);
+    GRAY_HIP(hipGetLastError());
+    return GRAY_OK;
+}
+
 // Reads (and clears) the device-side error flag; synchronises the stream.
 extern "C" int gray_device_status(gray_ctx *ctx, void *stream) {
     if (!ctx) return GRAY_ERR_INVALID;

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 from . import _lib as L
@@ -329,6 +330,7 @@ class NativeRaytracer:
         self.timer = _NoTimer()
         self._steps = 0
         self._built_count = -1
+        self._frame_prepared = False  # prepare_frame made the forward's resets
         self._forward_saved = False
         if num_gaussians > 0:
             self.resize(num_gaussians)
@@ -403,12 +405,41 @@ class NativeRaytracer:
         self.timer.stop(tk)
 
     # ---- passes -----------------------------------------------------------
+    def prepare_frame(self, origin, rotation_c2w, vertical_fov: float, znear: float, zfar: float) -> None:
+        """One launch for the holder updates before a frame: the camera fills
+        and set_pose of the Python layer (gray/raytracer.py:125-131, host
+        values here), rays_from_python cleared, and the metadata update and
+        was_visible reset that forward_pass would make
+        (raytracer.h:88-93).  The next forward_pass skips those two."""
+        cam, g = self.camera_data, self.gaussian_data
+        a = L.FrameArgs()
+        a.origin, a.rotation_c2w, a.rotation_w2c = (cam.origin.data_ptr(), cam.rotation_c2w.data_ptr(),
+                                                    cam.rotation_w2c.data_ptr())
+        a.vertical_fov, a.znear, a.zfar = (cam.vertical_fov_radians.data_ptr(), cam.znear.data_ptr(),
+                                           cam.zfar.data_ptr())
+        a.rays_from_python = self.config_data.rays_from_python.data_ptr()
+        a.grads_enabled = self.meta_data.grads_enabled.data_ptr()
+        a.total_num_calls = self.meta_data.total_num_calls.data_ptr()
+        a.was_visible = g.was_visible.data_ptr() if g.was_visible.numel() > 0 else None
+        a.count = g.was_visible.numel()
+        o = np.asarray(origin, dtype=np.float32).reshape(3)
+        rot = np.asarray(rotation_c2w, dtype=np.float32).reshape(9)
+        a.origin_value[:] = o.tolist()
+        a.c2w_value[:] = rot.tolist()
+        a.fov_value, a.znear_value, a.zfar_value = float(vertical_fov), float(znear), float(zfar)
+        a.grads_enabled_value = 1 if torch.is_grad_enabled() else 0
+        L.check(L.LIB.gray_frame_setup(C.byref(a), self._stream()))
+        self._frame_prepared = True
+
     def forward_pass(self) -> None:
         """Reference raytracer.h:88-102."""
         self._require_bvh()
-        self.meta_data.update()
+        if self._frame_prepared:
+            self._frame_prepared = False
+        else:
+            self.meta_data.update()
+            self.gaussian_data.was_visible.fill_(False)
         save = torch.is_grad_enabled() and not self.inference_only
-        self.gaussian_data.was_visible.fill_(False)
         self._sh_forward()
         if not save and self._ellipsoids_on():
             self._render_ellipsoids()

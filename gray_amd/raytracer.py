@@ -77,18 +77,14 @@ class Raytracer(torch.nn.Module):
 
     def __call__(self, cam_info: CameraInfo, znear=0.0, zfar=99999.9, skip_copy=False):
         "Render the scene from cam_info; returns [C, h, w]."
-        camera = self.cuda_module.get_camera()
-        camera.znear.fill_(znear)
-        camera.zfar.fill_(zfar)
-        config = self.cuda_module.get_config()
-        config.rays_from_python.fill_(False)
-        camera.vertical_fov_radians.fill_(float(cam_info.fov_y))
-        camera.set_pose(cam_info.origin_cuda(), cam_info.rotation_c2w_blender_cuda())
-
         framebuffer = self.cuda_module.get_framebuffer()
         grad_enabled = torch.is_grad_enabled()
         assert not (skip_copy and grad_enabled), "skip_copy=True is not supported with gradients enabled"
 
+        # the camera fills and set_pose (gray/raytracer.py:125-131) and the
+        # forward's resets, in one launch
+        self.cuda_module.prepare_frame(cam_info.origin, cam_info.rotation_c2w_blender(), float(cam_info.fov_y),
+                                       znear, zfar)
         self.cuda_module.forward_pass()
 
         h, w = self.render_height, self.render_width

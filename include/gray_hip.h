@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 8
+#define GRAY_ABI_VERSION 9
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -210,6 +210,31 @@ int gray_abi_version(void);
  * clears the device-side error flag (e.g. GRAY_ERR_OVERFLOW for a traversal
  * stack overflow) of every launch since the previous call. */
 int gray_device_status(gray_ctx *ctx, void *stream);
+
+/* No reference counterpart: the per-frame holder updates that precede a
+ * forward pass, in one launch instead of about ten small ones.  They are the
+ * camera fills and Camera::set_pose of the Python layer
+ * (gray/raytracer.py:125-131) and the metadata update and was_visible reset
+ * of Raytracer::forward_pass (cuda/raytracer.h:88-93).  Each device pointer
+ * may be NULL, which leaves that field alone. */
+typedef struct gray_frame_args {
+    float *origin;             /* device [3] <- origin_value */
+    float *rotation_c2w;       /* device [3,3] <- c2w_value (row-major) */
+    float *rotation_w2c;       /* device [3,3] <- the transpose of c2w_value */
+    float *vertical_fov;       /* device [1] <- fov_value (radians) */
+    float *znear;              /* device [1] <- znear_value */
+    float *zfar;               /* device [1] <- zfar_value */
+    uint8_t *rays_from_python; /* device bool [1] <- false */
+    uint8_t *grads_enabled;    /* device bool [1] <- grads_enabled_value != 0 */
+    int32_t *total_num_calls;  /* device [1] += 1 */
+    uint8_t *was_visible;      /* device bool [count] <- false */
+    int64_t count;
+    float origin_value[3];
+    float c2w_value[9];
+    float fov_value, znear_value, zfar_value;
+    int32_t grads_enabled_value;
+} gray_frame_args;
+int gray_frame_setup(const gray_frame_args *args, void *stream);
 
 /* ---- geometry + BVH ------------------------------------------------------- */
 

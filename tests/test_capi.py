@@ -46,3 +46,40 @@ def test_python_binding_matches_header():
     rc = _lib.LIB.gray_bvh_update(None, None, None, 1, None)
     assert rc == _lib.GRAY_ERR_INVALID
     assert b"null" in _lib.LIB.gray_last_error()
+
+
+MIRRORS = {"FrameArgs": "gray_frame_args", "Knobs": "gray_knobs", "Params": "gray_params", "View": "gray_view",
+           "RenderArgs": "gray_render_args", "Grads": "gray_grads", "Adam": "gray_adam",
+           "LossArgs": "gray_loss_args"}
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libgray_hip.so not built")
+def test_ctypes_structures_match_the_c_layout(tmp_path):
+    """Size and every field offset of the binding's structures, against the
+    header compiled by the host C compiler."""
+    import shutil
+    import subprocess
+
+    from gray_amd import _lib
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "gray_hip.h"', "int main(void) {"]
+    for py, c in MIRRORS.items():
+        cls = getattr(_lib, py)
+        lines.append(f'printf("{py} size %zu\\n", sizeof({c}));')
+        for name, *_ in cls._fields_:
+            lines.append(f'printf("{py} {name} %zu\\n", offsetof({c}, {name}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(p[0], p[1]): int(p[2]) for p in (l.split() for l in out if l)}
+    for py in MIRRORS:
+        cls = getattr(_lib, py)
+        assert got[(py, "size")] == ctypes.sizeof(cls), py
+        for name, *_ in cls._fields_:
+            assert got[(py, name)] == getattr(cls, name).offset, (py, name)
