@@ -353,6 +353,7 @@ class RemoteViewer:
         self.scaling_request: Optional[float] = None
         self.sessions: Dict[int, Tuple[ViewerSession, object]] = {}
         self.frames_sent = 0
+        self.error: Optional[BaseException] = None  # what stopped the render pump
         self._next_sid = 0
         self._pump_task: Optional[asyncio.Task] = None
         self._runner = None
@@ -397,7 +398,21 @@ class RemoteViewer:
     async def pump(self) -> None:
         """Submits a request for every session whose state changed (every
         session in continuous mode) with no frame in flight, polls the
-        scene, and sends finished frames."""
+        scene, and sends finished frames.  An error of the scene server
+        (a device error, say) closes every connection and ends the pump with
+        that error."""
+        try:
+            await self._pump()
+        except asyncio.CancelledError:
+            raise
+        except Exception as e:
+            self.error = e
+            print(f"ERROR: remote viewer render loop stopped: {e}", flush=True)
+            for _, ws in list(self.sessions.values()):
+                await ws.close(message=b"server error")
+            raise
+
+    async def _pump(self) -> None:
         loop = asyncio.get_running_loop()
         self._wake = asyncio.Event()
         while True:
@@ -459,16 +474,23 @@ class RemoteViewer:
         return site._server.sockets[0].getsockname()[1]
 
     async def stop(self) -> None:
+        """Stops the pump and the server, then re-raises the error that
+        stopped the pump, if one did."""
+        err = None
         if self._pump_task is not None:
             self._pump_task.cancel()
             try:
                 await self._pump_task
             except asyncio.CancelledError:
                 pass
+            except Exception as e:  # the pump's own error, already reported
+                err = e
             self._pump_task = None
         if self._runner is not None:
             await self._runner.cleanup()
             self._runner = None
+        if err is not None:
+            raise err
 
     def run(self, host: str = "0.0.0.0", port: int = 6009) -> None:
         """Serves until interrupted."""

@@ -250,3 +250,29 @@ def test_request_modes():
         assert Request(0, cam, mode=m).mode == m
     with pytest.raises(ValueError, match="display mode"):
         Request(0, cam, mode="Normals")
+
+
+class FailingScene(StandInScene):
+    def poll(self):
+        if self.queue:
+            raise RuntimeError("device: error flag 0x2")
+        return []
+
+
+def test_scene_error_closes_connections():
+    aiohttp = pytest.importorskip("aiohttp")
+    viewer = R.RemoteViewer(FailingScene(), scenes.tiny_camera(8, 8), codec="raw")
+
+    async def scenario():
+        port = await viewer.start("127.0.0.1", 0)
+        try:
+            async with aiohttp.ClientSession() as s:
+                c = await R.ViewerClient.connect(s, f"ws://127.0.0.1:{port}/")
+                msg = await c.ws.receive(timeout=10)
+                assert msg.type in (aiohttp.WSMsgType.CLOSE, aiohttp.WSMsgType.CLOSED)
+        finally:
+            with pytest.raises(RuntimeError, match="error flag"):
+                await viewer.stop()
+        assert "error flag" in str(viewer.error)
+
+    asyncio.run(asyncio.wait_for(scenario(), 60))
