@@ -139,6 +139,9 @@ struct TraceArgs {
     // wave_list_min (< 0: none), queued by the per-lane compositing kernel
     int32_t *heavy_count;
     uint32_t *heavy_rays;
+    // lists on: set by the per-lane compositing kernel when some ray's list
+    // overflowed, so the re-traversal kernel has work (NULL: lists off)
+    int32_t *retrace_any;
     int32_t wave_list_min, pad5;
 };
 

@@ -768,6 +768,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
     __shared__ float s_ka[GRAY_KB * GRAY_WAVE];
     __shared__ uint32_t s_stack[GRAY_STACK * GRAY_WAVE];
 
+    if (A.list_cap > 0 && A.retrace_any && *A.retrace_any == 0) return;  // no list overflowed
     int vi, tx, ty, tile;
     if (!block_tile(A, vi, tx, ty, &tile)) return;
     const DevView &V = A.views[vi];
@@ -1017,6 +1018,9 @@ __global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) 
     if (x < V.width && y < V.height) active = make_ray(A.knobs, V, x, y, r);
     const size_t lg = (size_t)tile * GRAY_WAVE + lane;
     const int n = A.list_n[lg];
+    // a ray whose list overflowed is re-traversed by k_trace_forward, which
+    // returns at once unless some wave raised this flag
+    if (__ballot(active && n < 0) && lane == 0) atomicOr(A.retrace_any, 1);
     active = active && n >= 0;
     // long lists go to the wave-per-ray kernel (one queue slot per lane,
     // taken with one atomic per wave)
@@ -2020,6 +2024,7 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
     A.error_flag = ctx->err;
     A.heavy_count = nullptr;
     A.heavy_rays = nullptr;
+    A.retrace_any = nullptr;
     A.wave_list_min = -1;
     return GRAY_OK;
 }
@@ -2168,6 +2173,7 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             A.heavy_count = ctx->task_counters + 2 * kTaskRounds + 2;
             A.heavy_rays = reinterpret_cast<uint32_t *>(ctx->tasks[0]);
             A.wave_list_min = args->wave_list_min == 0 ? GRAY_WAVE_LIST_MIN : args->wave_list_min;
+            A.retrace_any = ctx->task_counters + 2 * kTaskRounds + 3;  // zeroed with the task counters
             hipLaunchKernelGGL(k_composite_lists, grid, block, 0, s, A, A.views[0].rgb);
             GRAY_HIP(hipGetLastError());
             if (A.wave_list_min >= 0) {
