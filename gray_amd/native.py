@@ -407,7 +407,7 @@ class NativeRaytracer:
     # ---- passes -----------------------------------------------------------
     def prepare_frame(self, origin, rotation_c2w, vertical_fov: float, znear: float, zfar: float) -> None:
         """One launch for the holder updates before a frame: the camera fills
-        and set_pose of the Python layer (gray/raytracer.py:125-131, host
+        and set_pose of the Python layer (gray/raytracer.py:126-132, host
         values here), rays_from_python cleared, and the metadata update and
         was_visible reset that forward_pass would make
         (raytracer.h:88-93).  The next forward_pass skips those two."""

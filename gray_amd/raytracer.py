@@ -81,7 +81,7 @@ class Raytracer(torch.nn.Module):
         grad_enabled = torch.is_grad_enabled()
         assert not (skip_copy and grad_enabled), "skip_copy=True is not supported with gradients enabled"
 
-        # the camera fills and set_pose (gray/raytracer.py:125-131) and the
+        # the camera fills and set_pose (gray/raytracer.py:126-132) and the
         # forward's resets, in one launch
         self.cuda_module.prepare_frame(cam_info.origin, cam_info.rotation_c2w_blender(), float(cam_info.fov_y),
                                        znear, zfar)

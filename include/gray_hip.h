@@ -214,7 +214,7 @@ int gray_device_status(gray_ctx *ctx, void *stream);
 /* No reference counterpart: the per-frame holder updates that precede a
  * forward pass, in one launch instead of about ten small ones.  They are the
  * camera fills and Camera::set_pose of the Python layer
- * (gray/raytracer.py:125-131) and the metadata update and was_visible reset
+ * (gray/raytracer.py:126-132) and the metadata update and was_visible reset
  * of Raytracer::forward_pass (cuda/raytracer.h:88-93).  Each device pointer
  * may be NULL, which leaves that field alone. */
 typedef struct gray_frame_args {

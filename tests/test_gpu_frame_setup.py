@@ -1,5 +1,5 @@
 """gray_frame_setup: the one-launch holder updates before a frame (camera
-fills and set_pose of gray/raytracer.py:125-131, the metadata update and
+fills and set_pose of gray/raytracer.py:126-132, the metadata update and
 was_visible reset of raytracer.h:88-93) write exactly what the torch
 operations they replace write, and the reset covers any base alignment and
 length."""
