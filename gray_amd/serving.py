@@ -17,7 +17,7 @@ Plain mode is the comparison: one request per launch, in arrival order, on a
 single slot.  `submit`/`poll` drive the same admission without a clock of
 their own, for request sources that arrive over time (the remote viewer).
 
-A request's display mode is the reference viewer's (view.py:163-190):
+A request's display mode is the reference viewer's (view.py:153-190):
 "Gaussians" (the image), "Depth" (the image and the depth output) or
 "Ellipsoids" (the ellipsoid display, its own launch).  Requests of one batch
 share the launch kind, and ellipsoid batches their minimum opacity.
@@ -138,7 +138,7 @@ class SceneServer:
         return L.Knobs(*(k.data_ptr() for k in (self._k_depth if depth else self._k)))
 
     def set_global_scale_factor(self, value: float) -> None:
-        """The viewer's scaling modifier (view.py:171-176, 403-406): waits for
+        """The viewer's scaling modifier (view.py:171-176, 407-410): waits for
         the batches in flight (they keep their images, made with the old
         factor; poll() still returns them) and rebuilds every slot's BVH."""
         if float(self._k[3].item()) == float(value):
