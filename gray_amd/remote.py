@@ -210,7 +210,7 @@ def widget_state(camera: CameraInfo, z_near: float = 0.001, z_far: float = 100.0
 # ---- one connection ----------------------------------------------------------
 
 class ViewerSession:
-    """Server-side state of one client (view.py:97-150, 381-421)."""
+    """Server-side state of one client (view.py:97-150, 351-421)."""
 
     def __init__(self, viewer: "RemoteViewer", sid: int):
         self.viewer = viewer
@@ -248,7 +248,7 @@ class ViewerSession:
             self.apply(unpack_packet(frame))
 
     def apply(self, packet: Dict) -> None:
-        """view.py:408-421 (viewer state) and the camera widget's server_recv
+        """view.py:406-421 (viewer state) and the camera widget's server_recv
         (widgets/cameras/__init__.py:43-50)."""
         before = self._key()
         meta = packet.get("viewer", {}).get("metadata")
