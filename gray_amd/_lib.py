@@ -90,6 +90,7 @@ class Adam(C.Structure):
         ("numel", i64 * 7), ("beta1", vp), ("beta2", vp), ("epsilon", vp),
         ("was_visible", vp), ("pruning_counter", vp), ("count", i32),
         ("update_channels", vp), ("zero_grads", vp), ("enable_sh", vp), ("sh_update_laziness", vp),
+        ("step_device", vp),
     ]
 
 
@@ -145,7 +146,7 @@ def _load():
     return lib
 
 
-ABI_VERSION = 9  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
+ABI_VERSION = 10  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
 LIB = _load()
 if LIB.gray_abi_version() != ABI_VERSION:
     raise ImportError(f"libgray_hip.so has ABI {LIB.gray_abi_version()}, the binding expects {ABI_VERSION}; "

@@ -31,6 +31,7 @@ struct AdamArgs {
     float step_plus_one;
     const uint8_t *d_update_channels, *d_zero_grads, *d_enable_sh;
     const int32_t *d_laziness;
+    const int32_t *d_step;  // device step count (replaces step when set)
 };
 
 // Segment mode from the host flags or the device-resident switches.
@@ -49,13 +50,15 @@ __device__ __forceinline__ void store_stream(const float4 &v, float *p) {
     __builtin_nontemporal_store(x, reinterpret_cast<f4v *>(p));
 }
 
+__device__ __forceinline__ int step_of(const AdamArgs &A) { return A.d_step ? *A.d_step : A.step; }
+
 __device__ __forceinline__ int seg_mode(const AdamArgs &A, int t, bool &zero) {
     const bool upd_ch = A.d_update_channels ? *A.d_update_channels != 0 : (A.host_flags & 1) != 0;
     zero = A.d_zero_grads ? *A.d_zero_grads != 0 : (A.host_flags & 2) != 0;
     bool upd_sh;
     if (A.d_enable_sh) {
         const int lazy = A.d_laziness ? max(*A.d_laziness, 1) : 1;
-        upd_sh = *A.d_enable_sh != 0 && (A.step % lazy) == 0;
+        upd_sh = *A.d_enable_sh != 0 && (step_of(A) % lazy) == 0;
     } else {
         upd_sh = (A.host_flags & 4) != 0;
     }
@@ -105,8 +108,10 @@ __global__ __launch_bounds__(kThreads) void k_adam(AdamArgs A) {
     }
     const float lr = *A.lr[t];
     const float b1 = *A.beta1, b2 = *A.beta2, eps = *A.eps;
-    const float bc1 = 1.0f - powf(b1, A.step_plus_one);
-    const float bc2 = 1.0f - powf(b2, A.step_plus_one);
+    // adam.cu:16-17: powf(beta, step + 1.0f) with the step converted to float first
+    const float step1 = A.d_step ? (float)*A.d_step + 1.0f : A.step_plus_one;
+    const float bc1 = 1.0f - powf(b1, step1);
+    const float bc2 = 1.0f - powf(b2, step1);
 #pragma unroll
     for (int j = 0; j < kVecPerThread; ++j) {
         const int64_t i = base + ((int64_t)j * kThreads + threadIdx.x) * 4;
@@ -182,6 +187,7 @@ extern "C" int gray_adam_step(gray_ctx *ctx, const gray_adam *adam, int32_t step
     A.d_zero_grads = adam->zero_grads;
     A.d_enable_sh = adam->enable_sh;
     A.d_laziness = adam->sh_update_laziness;
+    A.d_step = adam->step_device;
     if (blocks == 0) return GRAY_OK;
     hipLaunchKernelGGL(k_adam, dim3(blocks), dim3(kThreads), 0, (hipStream_t)stream, A);
     GRAY_HIP(hipGetLastError());

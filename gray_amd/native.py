@@ -69,7 +69,10 @@ class ConfigDataHolder:
         self.eps_scale_grad = t(1e-12)
         self.background_channels = torch.zeros(L.NUM_CHANNELS, dtype=F32, device=device)
         self.render_depth = b(False)
-        self.render_ellipsoids = b(False)
+        # read only by the host (the display switch and the no-gradient check
+        # of Raytracer.__call__), so it lives in host memory: reading it costs
+        # no device synchronisation; fill_/copy_ work on it as on the others
+        self.render_ellipsoids = torch.tensor([False], dtype=torch.bool)
         self.ellipsoid_min_opacity = t(0.0)
         self.rays_from_python = b(False)
         self.zero_grads = b(True)
@@ -133,6 +136,8 @@ class GaussianDataHolder:
         self.beta_2 = torch.tensor([0.999], dtype=F32, device=device)
         self.epsilon = torch.tensor([1e-15], dtype=F32, device=device)
         self.sh_update_laziness = torch.tensor([1], dtype=torch.int32, device=device)
+        # gradients, moments and pruning statistics stay at one element when
+        # inference_only: resize() leaves them alone (gaussians.h:132-197, 236-262)
         for prefix in ("grad_", "first_moment_", "second_moment_"):
             for name in self.PARAMS:
                 shape = tuple(getattr(self, name).shape)
@@ -171,9 +176,10 @@ class GaussianDataHolder:
             self.count = int(n)
             # resize_ keeps the prefix; elements it newly exposes are
             # uninitialised memory, so they are cleared (learning rates copy [0])
+            prefixes = ("",) if self.inference_only else ("", "grad_", "first_moment_", "second_moment_")
             for name in self.PARAMS:
                 shp = self._shape(name, n)
-                for prefix in ("", "grad_", "first_moment_", "second_moment_"):
+                for prefix in prefixes:
                     t = getattr(self, prefix + name)
                     t.resize_(shp)
                     if n > old:
@@ -183,11 +189,12 @@ class GaussianDataHolder:
                 t.resize_(n, 1)
                 if n > old:
                     t[old:].copy_(t[:1].expand(n - old, 1))
-            for name in ("pruning_weight", "pruning_counter", "was_visible", "marked"):
-                t = getattr(self, name)
-                t.resize_(n, 1)
-                if n > old:
-                    t[old:].zero_()
+            if not self.inference_only:
+                for name in ("pruning_weight", "pruning_counter", "was_visible", "marked"):
+                    t = getattr(self, name)
+                    t.resize_(n, 1)
+                    if n > old:
+                        t[old:].zero_()
         self._attach_grads()
 
     def increment_sh_degree(self) -> None:
@@ -328,7 +335,6 @@ class NativeRaytracer:
         self.wave_list_min = 0  # lists longer than this are composited by a whole wave (0: built-in, < 0: never)
         self.tile_stats = None  # optional int32 [tiles, 4] traversal statistics of the forward
         self.timer = _NoTimer()
-        self._steps = 0
         self._built_count = -1
         self._frame_prepared = False  # prepare_frame made the forward's resets
         self._forward_saved = False
@@ -440,6 +446,8 @@ class NativeRaytracer:
             self.meta_data.update()
             self.gaussian_data.was_visible.fill_(False)
         save = torch.is_grad_enabled() and not self.inference_only
+        if save and self._ellipsoids_on():
+            raise RuntimeError("render_ellipsoids=True is only supported for no-grad display renders")
         self._sh_forward()
         if not save and self._ellipsoids_on():
             self._render_ellipsoids()
@@ -462,9 +470,10 @@ class NativeRaytracer:
         L.check(L.LIB.gray_render_forward(self.ctx.handle, C.byref(self._args(view, False)), self._stream()))
 
     def _ellipsoids_on(self) -> bool:
-        """config.render_ellipsoids, read on the host only for renders without
-        gradients (the reference's display switch, shaders.cu:90-93; its
-        Python layer rejects it with gradients, gray/raytracer.py:142-144)."""
+        """config.render_ellipsoids (host memory, no synchronisation): the
+        reference's display switch (shaders.cu:90-93).  Its Python layer
+        rejects it with gradients (gray/raytracer.py:142-144), and so does
+        forward_pass here."""
         return bool(self.config_data.render_ellipsoids.item())
 
     def _render_ellipsoids(self) -> None:
@@ -527,10 +536,11 @@ class NativeRaytracer:
             raise RuntimeError("step is unavailable for an inference-only raytracer")
         with torch.no_grad():
             self.meta_data.total_num_steps.add_(1)
-        self._steps += 1
         a = self.adam_struct(update_pruning_stats)
+        # the kernel reads the incremented counter on the device (raytracer.h:127-128)
+        a.step_device = self.meta_data.total_num_steps.data_ptr()
         tk = self.timer.start("adam")
-        L.check(L.LIB.gray_adam_step(self.ctx.handle, C.byref(a), self._steps, 0, self._stream()))
+        L.check(L.LIB.gray_adam_step(self.ctx.handle, C.byref(a), 0, 0, self._stream()))
         self.timer.stop(tk)
 
     def update_bvh(self) -> None:

@@ -80,6 +80,10 @@ class Raytracer(torch.nn.Module):
         framebuffer = self.cuda_module.get_framebuffer()
         grad_enabled = torch.is_grad_enabled()
         assert not (skip_copy and grad_enabled), "skip_copy=True is not supported with gradients enabled"
+        # render_ellipsoids is a host-memory knob here: the check costs no synchronisation
+        assert not (
+            grad_enabled and self.cuda_module.get_config().render_ellipsoids.item()
+        ), "render_ellipsoids=True is only supported for no-grad display renders"
 
         # the camera fills and set_pose (gray/raytracer.py:126-132) and the
         # forward's resets, in one launch

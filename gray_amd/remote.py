@@ -435,7 +435,10 @@ class RemoteViewer:
                 ws = self.sessions[sess.sid][1]
                 if req.error is not None:
                     text, _ = pack_packet({"viewer": {"error": req.error}})
-                    await ws.send_str(text)
+                    try:
+                        await ws.send_str(text)
+                    except (ConnectionError, RuntimeError):
+                        sess.closed = True  # only this client's socket is affected
                     continue
                 rgb8 = sess.frame_rgb8(req)
                 text, blob = await loop.run_in_executor(None, sess.frame_packet, rgb8)

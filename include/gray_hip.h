@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 9
+#define GRAY_ABI_VERSION 10
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -175,6 +175,12 @@ typedef struct {
     const uint8_t *zero_grads;         /* [1] bool or NULL */
     const uint8_t *enable_sh;          /* [1] bool or NULL */
     const int32_t *sh_update_laziness; /* [1] or NULL */
+    /* device [1] or NULL: the step count after its increment (the reference's
+     * meta_data.total_num_steps, raytracer.h:127-128); when set it replaces
+     * the host `step` argument of gray_adam_step for the bias correction and
+     * the SH laziness test, so a caller that restores or edits the counter
+     * steers the optimiser without a device->host read */
+    const int32_t *step_device;
 } gray_adam;
 
 /* Training loss over P strided fp32 planes of H x W pixels (render and
@@ -295,9 +301,24 @@ int gray_sh_backward(gray_ctx *ctx, const gray_params *params, const gray_knobs 
 /* Replaces the forward half of Raytracer::forward_pass (raytracer.h:88-102):
  * __raygen__rg + __intersection__is + forward_pass (shaders.cu:10-106,
  * forward_pass.cu:6-151).  Needs a prior gray_bvh_update with the same N.
- * Wavefront-coherent BVH traversal (one 8x8 ray tile per wave64), analytic
- * ray-ellipsoid test, per-ray sorted 32-entry k-buffer in LDS, front-to-back
- * compositing in rounds of 32 with the reference truncation estimate. */
+ * Views are processed in launches of up to 8; rays in 8x8 tiles, one wave64
+ * per tile.  With list_capacity > 0 (the default path):
+ *   1. collect: each lane walks the 4-wide BVH for its own ray (fp16 child
+ *      boxes, short stack in LDS, leaves queued in LDS), runs the exact
+ *      ray-ellipsoid test (shaders.cu:22-47) and appends every hit to the
+ *      ray's list in the context (16 B per hit, slot-major); after a budget
+ *      of iterations the rest of a walk is handed on as {ray, subtree} tasks
+ *      that persistent task rounds take one per lane;
+ *   2. composite: one lane per ray scans its list with a 16-entry register
+ *      k-buffer, in the reference's batches of 32 beyond tmin (sub-rounds of
+ *      16), front to back, stopping at T < t_threshold; lists longer than
+ *      wave_list_min (up to 512 hits) are composited by a whole wave each;
+ *   3. rays whose list or tasks overflowed are re-traversed per batch by a
+ *      k-buffer kernel (16 entries in LDS), as is every ray when
+ *      list_capacity == 0.
+ * Cost model: the collect walk is bound by the dependent latency of its
+ * node visits (about 95 4-wide node visits and 44 leaf tests per ray on a
+ * 1M-Gaussian surface scene); compositing reads each list once. */
 int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, void *stream);
 
 /* Replaces the ellipsoid display mode of the forward launch (config
@@ -323,8 +344,9 @@ int gray_render_backward(gray_ctx *ctx, const gray_render_args *args, const gray
 /* Replaces adam_step + adam_step_base_kernel + adam_step_sh_kernel
  * (cuda/opt/adam.cu:25-84, raytracer.h:126-133) and the pruning counter update
  * of Raytracer.update_pruning_stats (gray/raytracer.py:361-366), in ONE
- * launch.  step is total_num_steps after the increment; the bias correction
- * uses step+1 like the reference (adam.cu:16-17).  flags: bit0 update_channels,
+ * launch.  step is total_num_steps after the increment (or *adam->step_device
+ * when given); the bias correction uses step+1 like the reference
+ * (adam.cu:16-17).  flags: bit0 update_channels,
  * bit1 zero_grads, bit2 update SH (enable_sh && step % sh_update_laziness == 0);
  * each is replaced by the device-resident switch in gray_adam when given. */
 int gray_adam_step(gray_ctx *ctx, const gray_adam *adam, int32_t step, int32_t flags, void *stream);
