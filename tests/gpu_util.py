@@ -74,3 +74,26 @@ def rel_err(a, b):
 
 def deg(x):
     return math.radians(x)
+
+
+def per_element_err(a, b, atol_frac=1e-3):
+    """Worst per-element relative error: max over elements of
+    |a - b| / (|b| + atol_frac * max|b|).  A bound of 1e-3 on it means
+    |a - b| <= 1e-3 |b| + 1e-6 max|b| element by element, so small gradient
+    components are held to the relative bar, not just the largest one."""
+    a = np.asarray(a, np.float64).reshape(-1)
+    b = np.asarray(b, np.float64).reshape(-1)
+    floor = atol_frac * max(np.abs(b).max(), 1e-30)
+    return float((np.abs(a - b) / (np.abs(b) + floor)).max())
+
+
+def oracle_knobs(nr):
+    """Oracle knobs from a NativeRaytracer's config and camera holders."""
+    from oracle import oracle as O
+
+    c = nr.get_config()
+    cam = nr.get_camera()
+    return O.make_knobs(exp_power=float(c.exp_power), alpha_threshold=float(c.alpha_threshold),
+                        t_threshold=float(c.t_threshold), eps_fwd=float(c.eps_forward_normalization),
+                        eps_scale_grad=float(c.eps_scale_grad), bg=tuple(c.background_channels.tolist()),
+                        znear=float(cam.znear), zfar=float(cam.zfar), render_depth=bool(c.render_depth))
