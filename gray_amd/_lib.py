@@ -138,6 +138,7 @@ def _load():
         "gray_loss_workspace_bytes": (i32, [i32, i32, i32, C.POINTER(i64)]),
         "gray_loss_forward": (i32, [C.POINTER(LossArgs), vp]),
         "gray_loss_backward": (i32, [C.POINTER(LossArgs), vp]),
+        "gray_loss_noalloc": (i32, [C.POINTER(LossArgs), vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -157,8 +158,9 @@ EXPORTED = (
     "gray_sh_backward", "gray_render_forward", "gray_render_backward", "gray_adam_step", "gray_debug_hitlists",
     "gray_debug_snapshot", "gray_debug_bvh", "gray_debug_list_counts", "gray_render_stage_ms",
     "gray_render_reserve", "gray_loss_workspace_bytes", "gray_loss_forward", "gray_loss_backward",
-    "gray_render_ellipsoids", "gray_debug_wide", "gray_frame_setup",
+    "gray_render_ellipsoids", "gray_debug_wide", "gray_frame_setup", "gray_loss_noalloc",
 )
+LOSS_NOALLOC_MAX_BYTES = 80 << 20  # GRAY_LOSS_NOALLOC_MAX_BYTES
 
 
 class GrayError(RuntimeError):

@@ -31,7 +31,12 @@
 
 #include <math.h>
 
+#include <mutex>
+
 #include "ctx.hpp"
+
+// The workspace of gray_loss_noalloc (see static_workspace below).
+static __device__ __align__(256) unsigned char scratch[GRAY_LOSS_NOALLOC_MAX_BYTES];
 
 namespace {
 
@@ -374,14 +379,107 @@ int setup(const gray_loss_args *a, LossArgs &A) {
     return GRAY_OK;
 }
 
+int launch_forward(const LossArgs &A, hipStream_t s) {
+    hipLaunchKernelGGL(k_loss_forward, dim3(8 * A.xcd_chunk, A.groups), dim3(kThreads), 0, s, A);
+    GRAY_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(kFinishThreads), 0, s, A);
+    GRAY_HIP(hipGetLastError());
+    return GRAY_OK;
+}
+
+int launch_backward(const LossArgs &A, hipStream_t s) {
+    hipLaunchKernelGGL(k_loss_backward, dim3(8 * A.xcd_chunk, A.groups), dim3(kThreads), 0, s, A);
+    GRAY_HIP(hipGetLastError());
+    return GRAY_OK;
+}
+
+int64_t workspace_bytes(int64_t P, int64_t H, int64_t W) {
+    return maps_bytes(P, H, W) + 2 * 4 * ((P + kGroup - 1) / kGroup) * tiles_of(H, W);
+}
+
+// Workspace of gray_loss_noalloc: one file-scope array per GPU, held from the
+// library's load until the process ends.  Host code never uses the name as an
+// address (on the host it is only a stand-in for the symbol): the device
+// address of each GPU's copy comes from hipGetSymbolAddress with that device
+// current, on the first call on that device, and only that address reaches the
+// kernels.  Nothing in it has to start at zero: the forward writes every
+// partial sum and every derivative map entry that the finish and backward
+// kernels read.
+constexpr int64_t kStaticWorkspaceBytes = GRAY_LOSS_NOALLOC_MAX_BYTES;
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int static_workspace(void **out) {
+    int dev = -1;
+    GRAY_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        gray_set_error(GRAY_ERR_INVALID, "loss_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+        return GRAY_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return gray_check_hip(e, "loss_noalloc: hipGetSymbolAddress(scratch)");
+        if (!p) {
+            gray_set_error(GRAY_ERR_HIP, "loss_noalloc: null device address of the static workspace");
+            return GRAY_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GRAY_OK;
+}
+
 }  // namespace
+
+extern "C" int gray_loss_noalloc(const gray_loss_args *args, void *stream) {
+    if (!args) {
+        gray_set_error(GRAY_ERR_INVALID, "loss_noalloc: null args");
+        return GRAY_ERR_INVALID;
+    }
+    if (args->workspace) {
+        gray_set_error(GRAY_ERR_INVALID, "loss_noalloc: workspace must be NULL (the library holds it)");
+        return GRAY_ERR_INVALID;
+    }
+    if (args->planes <= 0 || args->height <= 0 || args->width <= 0) {
+        gray_set_error(GRAY_ERR_INVALID, "loss_noalloc: bad shape planes=%d height=%d width=%d", args->planes,
+                       args->height, args->width);
+        return GRAY_ERR_INVALID;
+    }
+    const int64_t need = workspace_bytes(args->planes, args->height, args->width);
+    if (need > kStaticWorkspaceBytes) {
+        gray_set_error(GRAY_ERR_INVALID,
+                       "loss_noalloc: %d planes of %dx%d need %lld workspace bytes, above the static %lld",
+                       args->planes, args->width, args->height, (long long)need, (long long)kStaticWorkspaceBytes);
+        return GRAY_ERR_INVALID;
+    }
+    void *ws = nullptr;
+    if (int rc = static_workspace(&ws)) return rc;
+    gray_loss_args a = *args;
+    a.workspace = ws;
+    a.train = args->grad ? 1 : 0;
+    LossArgs A;
+    if (int rc = setup(&a, A)) return rc;
+    if (!A.loss) {
+        gray_set_error(GRAY_ERR_INVALID, "loss_noalloc: null loss output");
+        return GRAY_ERR_INVALID;
+    }
+    const hipStream_t s = (hipStream_t)stream;
+    if (int rc = launch_forward(A, s)) return rc;
+    if (A.grad)
+        if (int rc = launch_backward(A, s)) return rc;
+    return GRAY_OK;
+}
 
 extern "C" int gray_loss_workspace_bytes(int32_t planes, int32_t height, int32_t width, int64_t *bytes) {
     if (!bytes || planes <= 0 || height <= 0 || width <= 0) {
         gray_set_error(GRAY_ERR_INVALID, "loss workspace: bad shape or null output");
         return GRAY_ERR_INVALID;
     }
-    *bytes = maps_bytes(planes, height, width) + 2 * 4 * (int64_t)((planes + kGroup - 1) / kGroup) * tiles_of(height, width);
+    *bytes = workspace_bytes(planes, height, width);
     return GRAY_OK;
 }
 
@@ -392,12 +490,7 @@ extern "C" int gray_loss_forward(const gray_loss_args *args, void *stream) {
         gray_set_error(GRAY_ERR_INVALID, "loss forward: null loss output");
         return GRAY_ERR_INVALID;
     }
-    const hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_loss_forward, dim3(8 * A.xcd_chunk, A.groups), dim3(kThreads), 0, s, A);
-    GRAY_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_loss_finish, dim3(1), dim3(kFinishThreads), 0, s, A);
-    GRAY_HIP(hipGetLastError());
-    return GRAY_OK;
+    return launch_forward(A, (hipStream_t)stream);
 }
 
 extern "C" int gray_loss_backward(const gray_loss_args *args, void *stream) {
@@ -407,7 +500,5 @@ extern "C" int gray_loss_backward(const gray_loss_args *args, void *stream) {
         gray_set_error(GRAY_ERR_INVALID, "loss backward: null gradient output");
         return GRAY_ERR_INVALID;
     }
-    hipLaunchKernelGGL(k_loss_backward, dim3(8 * A.xcd_chunk, A.groups), dim3(kThreads), 0, (hipStream_t)stream, A);
-    GRAY_HIP(hipGetLastError());
-    return GRAY_OK;
+    return launch_backward(A, (hipStream_t)stream);
 }

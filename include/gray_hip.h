@@ -369,6 +369,21 @@ int gray_loss_forward(const gray_loss_args *args, void *stream);
  * l1_loss. */
 int gray_loss_backward(const gray_loss_args *args, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving or training loops,
+ * harnesses that forbid hipMalloc): the loss of gray_loss_forward and, when
+ * args->grad is not NULL, the gradient of gray_loss_backward, in one call,
+ * with the workspace held by the library in a static device array of
+ * GRAY_LOSS_NOALLOC_MAX_BYTES per GPU (allocated when the library's code is
+ * loaded on that GPU, held until the process ends).  args->workspace must be
+ * NULL and args->train is ignored.  Shapes whose workspace
+ * (gray_loss_workspace_bytes) exceeds the maximum are rejected with
+ * GRAY_ERR_INVALID; the limit admits a 1920x1080 RGB image.  Calls no
+ * allocating routine.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (on different streams or host threads) race. */
+#define GRAY_LOSS_NOALLOC_MAX_BYTES (80ll << 20)
+int gray_loss_noalloc(const gray_loss_args *args, void *stream);
+
 /* ---- debugging / parity --------------------------------------------------- */
 
 /* No reference counterpart (the reference keeps hits in its per-pixel linked
