@@ -77,8 +77,16 @@ def hit_mask(mean, rotation, scale, opacity, ro, rd, exp_power=2.0, alpha_thresh
 
 
 def render(mean, rotation, scale, opacity, colors, ro, rd, exp_power=2.0, alpha_threshold=0.01, t_threshold=0.01,
-           bg=(0.0, 0.0, 0.0), znear=0.0, zfar=99999.9):
-    """Returns (image [R,3], dict of per-ray extras).  All inputs float64."""
+           bg=(0.0, 0.0, 0.0), znear=0.0, zfar=99999.9, fixed=None):
+    """Returns (image [R,3], dict of per-ray extras).  All inputs float64.
+
+    fixed: optional per-ray (composited, skipped) lists of Gaussian indices.
+    The hit structure (which Gaussians a ray composites, in which order, and
+    which enter the truncation estimate) is then taken as given, typically
+    from the GPU, and only the smooth weights are evaluated: finite
+    differences of this function stay within one piece of the piecewise-
+    smooth forward, whose pieces are set by float32 hit decisions and
+    distance order that float64 does not reproduce at ties."""
     dt = mean.dtype
     op = torch.sigmoid(opacity.reshape(-1))
     clip = clip_factor(op.detach(), alpha_threshold, exp_power)
@@ -110,6 +118,30 @@ def render(mean, rotation, scale, opacity, colors, ro, rd, exp_power=2.0, alpha_
     extras = {"T": [], "F": [], "ncomp": [], "order": []}
     bgt = torch.tensor(bg, dtype=dt)
     for r in range(Rn):
+        if fixed is not None:
+            comp, skipped = fixed[r]
+            T = torch.ones((), dtype=dt)
+            pix = torch.zeros(3, dtype=dt)
+            for gid in comp:
+                a = alpha[r, gid]
+                pix = pix + colors[gid] * (T * a)
+                T = T * (1 - a)
+            skip_c = torch.zeros(3, dtype=dt)
+            skip_a = torch.zeros((), dtype=dt)
+            skip_p = torch.ones((), dtype=dt)
+            for gid in skipped:
+                ad = alpha[r, gid].detach()
+                skip_c = skip_c + colors[gid].detach() * ad
+                skip_a = skip_a + ad
+                skip_p = skip_p * (1 - ad)
+            F = T * skip_p
+            est = skip_c / skip_a if float(skip_a) > 0 else torch.zeros(3, dtype=dt)
+            out[r] = pix + (T - F) * est + bgt * F
+            extras["T"].append(float(T))
+            extras["F"].append(float(F))
+            extras["ncomp"].append(len(comp))
+            extras["order"].append(list(comp))
+            continue
         idx = torch.nonzero(hit[r]).reshape(-1)
         if idx.numel() == 0:
             out[r] = bgt

@@ -2,7 +2,7 @@
 camera on a ring of radius 6) against the oracle's brute force over all 1M
 Gaussians on a seeded sample of pixels: hit lists bit-exact, image and
 transmittances within 1e-5 (same snapshot and rays), and the gradients of a
-loss that only those pixels see within 1e-3 relative."""
+loss that only those pixels see within 1e-3 relative, element by element."""
 import numpy as np
 import pytest
 import torch
@@ -10,7 +10,7 @@ import torch
 from gray_amd import scenes
 from oracle import oracle as O
 
-from gpu_util import rays_of, rel_err, setup_native
+from gpu_util import assert_grads_close, comparable_gaussians, rays_of, setup_native
 
 pytestmark = pytest.mark.gpu
 
@@ -88,6 +88,8 @@ def test_fullsize_backward_matches_oracle(frame):
     assert touched.sum() > 500
     # the pruning weight sums every composited hit of the frame whatever the
     # output gradient, so only the gradients are comparable on a sample
-    for name, key in (("grad_opacity", "opacity"), ("grad_scale", "scale"), ("grad_rotation", "rotation"),
-                      ("grad_channels", "rgb")):
-        assert rel_err(getattr(g, name).cpu(), ob[key]) < 1e-3, name
+    ncomp = nr.hits.n_comp[: W * H][torch.from_numpy(pix).cuda()].cpu().numpy()
+    ok = comparable_gaussians(ncomp, ref, geo, ro, rd)
+    assert_grads_close([(name, getattr(g, name).cpu(), ob[key]) for name, key in (
+        ("grad_opacity", "opacity"), ("grad_scale", "scale"), ("grad_rotation", "rotation"),
+        ("grad_channels", "rgb"))], ok)

@@ -13,7 +13,7 @@ from gray_amd.functional import MultiViewRenderer, RenderSettings
 from oracle import oracle as O
 from oracle import torch_ref
 
-from gpu_util import rel_err, setup_native
+from gpu_util import assert_grads_close, comparable_gaussians, rel_err, setup_native
 
 pytestmark = pytest.mark.gpu
 
@@ -55,9 +55,11 @@ def test_gradients_equal_summed_oracle_views():
     torch.cuda.synchronize()
     r.device_status()
     k = O.make_knobs(t_threshold=0.03)
-    geo, l2w = O.snapshot(scene.mean.numpy(), scene.rotation.numpy(), scene.scale.numpy(), scene.opacity.numpy(),
-                          0.01, 2.0)
+    _, l2w = O.snapshot(scene.mean.numpy(), scene.rotation.numpy(), scene.scale.numpy(), scene.opacity.numpy(),
+                        0.01, 2.0)
+    geo = r.debug_snapshot()[0].cpu().numpy()  # the GPU's snapshot: the same hit sets as the render
     tot = {key: 0 for key in ("mean", "rotation", "scale", "opacity", "sh_dc", "sh_rest")}
+    ok = np.ones(scene.count, bool)
     for i, cam in enumerate(cams):
         ro = r.ro[i].reshape(-1, 3).cpu().numpy()
         rd = r.rd[i].reshape(-1, 3).cpu().numpy()
@@ -65,6 +67,7 @@ def test_gradients_equal_summed_oracle_views():
         rgb = O.sh_forward(scene.mean.numpy(), scene.sh_dc.numpy(), scene.sh_rest.numpy(), origin, 3)
         fwd = O.render(geo, rgb, ro, rd, k)
         assert np.abs(imgs[i].detach().reshape(-1, 3).cpu().numpy() - fwd["C"]).max() < 1e-5
+        ok &= comparable_gaussians(r.n_comp[i].cpu().numpy(), fwd, geo, ro, rd)
         b = O.backward(geo, l2w, rgb, scene.rotation.numpy(), scene.scale.numpy(), scene.opacity.numpy(), ro, rd,
                        fwd, w[i].reshape(-1, 3).cpu().numpy(), k, 32 * 24)
         vis = ((np.abs(b["opacity"]).reshape(-1) > 0) | (b["pruning"].reshape(-1) > 0)).astype(np.uint8)
@@ -76,8 +79,8 @@ def test_gradients_equal_summed_oracle_views():
         tot["opacity"] = tot["opacity"] + b["opacity"]
         tot["sh_dc"] = tot["sh_dc"] + gdc
         tot["sh_rest"] = tot["sh_rest"] + grest
-    for name, p in zip(PARAMS, params):
-        assert rel_err(p.grad.cpu(), tot[name].reshape(p.shape)) < 1e-3, name
+    assert_grads_close([(name, p.grad.cpu().reshape(p.shape[0], -1), tot[name].reshape(p.shape[0], -1))
+                        for name, p in zip(PARAMS, params)], ok)
 
 
 def test_gradients_match_fp64_finite_differences():
@@ -155,7 +158,9 @@ def test_config3_scale_under_memory_cap():
     launch, hit lists under a 4 GB cap: composited lists longer than the
     derived store are recomputed in the backward.  Gradients equal those of an
     uncapped store, and 64 random parameters match float64 central
-    differences (h = 1e-4; t_threshold 0 as above)."""
+    differences (h = 1e-4; t_threshold 0 as above) taken with the GPU's hit
+    structure held fixed, half of them drawn from the largest fifth of the
+    Gaussians so that large footprints are covered; none is skipped."""
     scene = scenes.surface_scene(200_000, 20, -4.5, 0.4, seed=1)
     cams = scenes.ring_cameras(8, 6.0, 512, 512)
     capped = MultiViewRenderer(512, 512, RenderSettings(t_threshold=0.0, memory_cap_bytes=4 << 30))
@@ -176,82 +181,91 @@ def test_config3_scale_under_memory_cap():
     torch.cuda.synchronize()
     for name, a, b in zip(PARAMS, grads, params2):
         assert rel_err(a.cpu(), b.grad.cpu()) < 1e-4, name
-    # float64 central differences on 64 random parameters
-    _, aabb = capped.debug_snapshot()
-    aabb = aabb.cpu().numpy().astype(np.float64)
-    ro = torch.tensor(capped.ro.reshape(-1, 3).cpu().numpy(), dtype=torch.float64)
-    rd = torch.tensor(capped.rd.reshape(-1, 3).cpu().numpy(), dtype=torch.float64)
-    wv = torch.tensor(wgt.reshape(-1, 3).cpu().numpy(), dtype=torch.float64)
-    view_of = torch.arange(ro.shape[0]) // (512 * 512)
+    # float64 central differences on 64 random parameters, each within the
+    # GPU's piece of the piecewise-smooth forward: every ray keeps the hit
+    # structure the GPU composited (from the oracle on the GPU's snapshot and
+    # rays, which matches the GPU hit for hit); float64 alone would decide a
+    # few hits and near-equal distances differently on rays of large Gaussians
+    geo, aabb = capped.debug_snapshot()
+    geo = geo.cpu().numpy()
+    aabb = aabb.cpu().numpy()
+    ro32 = capped.ro.reshape(8, -1, 3).cpu().numpy()
+    rd32 = capped.rd.reshape(8, -1, 3).cpu().numpy()
+    wv = wgt.reshape(8, -1, 3).cpu().numpy().astype(np.float64)
     p64 = [getattr(scene, k).to(torch.float64) for k in PARAMS]
     origins = [torch.tensor(c.origin, dtype=torch.float64) for c in cams]
+    k = O.make_knobs(t_threshold=0.0)
 
-    # box prefilters in float64 on the device (test scaffolding only)
-    ro_g, inv_g = ro.cuda(), 1.0 / rd.cuda()
-    aabb_g = torch.tensor(aabb, device="cuda")
-
-    def slab(rays, boxes, pad):
-        """[len(rays), len(boxes)]: the ray enters the box grown by pad."""
-        o, inv = ro_g[rays][:, None, :], inv_g[rays][:, None, :]
-        t0 = (boxes[None, :, :3] - pad - o) * inv
-        t1 = (boxes[None, :, 3:] + pad - o) * inv
-        tf = torch.maximum(t0, t1).amin(-1)
-        return (torch.minimum(t0, t1).amax(-1) <= tf) & (tf >= 0)
-
-    all_rays = torch.arange(ro.shape[0], device="cuda")
-
-    def candidates(rays):
-        """Gaussians whose (conservative) box one of the rays enters."""
-        hit = torch.zeros(aabb.shape[0], dtype=torch.bool, device="cuda")
-        for c in range(0, len(rays), 8):
-            hit |= slab(rays[c:c + 8].cuda(), aabb_g, 1e-3).any(0)
-        return torch.nonzero(hit).reshape(-1).cpu().numpy()
-
-    def loss(pp, rays, sub):
-        tot = torch.zeros((), dtype=torch.float64)
-        q = [t[sub] for t in pp]
+    def rays_of_gaussian(j):
+        """Per view: the rays whose hit list holds j, their hit lists and composited lists."""
+        out = []
         for v in range(8):
-            idx = rays[view_of[rays] == v]
-            if idx.numel() == 0:
+            o, d = ro32[v], rd32[v]
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t0 = (aabb[j, :3] - o) / d
+                t1 = (aabb[j, 3:] - o) / d
+            tn = np.nanmax(np.minimum(t0, t1), axis=1)
+            tf = np.nanmin(np.maximum(t0, t1), axis=1)
+            near = np.nonzero((tn <= tf) & (tf >= 0))[0]
+            if near.size == 0:
                 continue
+            ids, _, counts = O.hitlists(geo, o[near], d[near], 0.01, 4096)
+            has = np.array([j in ids[i, :counts[i]] for i in range(near.size)], bool)
+            if not has.any():
+                continue
+            rays = near[has]
+            fwd = O.render(geo, capped.rgb[v].cpu().numpy(), o[rays], d[rays], k)
+            hits = [ids[i, :counts[i]] for i in np.nonzero(has)[0]]
+            comp = [fwd["ids"][i, :fwd["ncomp"][i]] for i in range(rays.size)]
+            out.append((v, rays, hits, comp))
+        return out
+
+    def loss(pp, per_view, sub):
+        pos = {int(g): i for i, g in enumerate(sub)}
+        q = [t[torch.from_numpy(sub)] for t in pp]
+        tot = torch.zeros((), dtype=torch.float64)
+        for v, rays, hits, comp in per_view:
+            fixed = []
+            for hl, cl in zip(hits, comp):
+                cset = set(int(g) for g in cl)
+                fixed.append(([pos[int(g)] for g in cl], [pos[int(g)] for g in hl if int(g) not in cset]))
             col = torch_ref.sh_colors(q[0], q[4], q[5], origins[v])
-            img, _ = torch_ref.render(q[0], q[1], q[2], q[3], col, ro[idx], rd[idx], 2.0, 0.01, 0.0)
-            tot = tot + (img * wv[idx]).sum()
+            img, _ = torch_ref.render(q[0], q[1], q[2], q[3], col, torch.from_numpy(ro32[v][rays]).double(),
+                                      torch.from_numpy(rd32[v][rays]).double(), 2.0, 0.01, 0.0, fixed=fixed)
+            tot = tot + (img * torch.from_numpy(wv[v][rays])).sum()
         return tot
 
     rng = np.random.default_rng(11)
     vis = np.nonzero(params[3].grad.abs().reshape(-1).cpu().numpy() > 0)[0]
+    # stratified: every other draw from the largest fifth of the visible
+    # Gaussians (by largest axis), so Gaussians seen by hundreds of rays are
+    # checked as well as small ones
+    size = scene.scale.max(dim=1).values.numpy()[vis]
+    big = vis[np.argsort(size)[-max(1, len(vis) // 5):]]
     h = 1e-4
-    checked, worst = 0, 0.0
-    for _ in range(400):
-        if checked == 64:
-            break
-        j = int(rng.choice(vis))
+    checked, worst, many_rays = 0, 0.0, 0
+    for trial in range(64):
+        j = int(rng.choice(big if trial % 2 else vis))
         which = int(rng.integers(0, 6))
         comp = int(rng.integers(0, p64[which][j].numel()))
-        near = torch.nonzero(slab(all_rays, aabb_g[j:j + 1], 1e-3)[:, 0]).reshape(-1).cpu()
-        if near.numel() == 0:
-            continue
-        one = lambda pp: torch_ref.hit_mask(*[t[j:j + 1] for t in pp[:4]], ro[near], rd[near])[:, 0]  # noqa: E731
-        base = one(p64)
+        per_view = rays_of_gaussian(j)
+        n_rays = sum(len(pv[1]) for pv in per_view)
+        assert n_rays > 0, j
+        sub = np.unique(np.concatenate([np.concatenate(pv[2]) for pv in per_view] + [np.array([j])]))
         pert = []
         for sign in (1.0, -1.0):
             pp = [t.clone() for t in p64]
             pp[which][j].view(-1)[comp] += sign * h
             pert.append(pp)
-        if any(not torch.equal(one(pp), base) for pp in pert):
-            continue
-        rays = near[base]
-        if rays.numel() == 0 or rays.numel() > 64:
-            continue
-        sub = torch.tensor(np.union1d(candidates(rays), [j]), dtype=torch.long)
-        fd = float(loss(pert[0], rays, sub) - loss(pert[1], rays, sub)) / (2 * h)
+        fd = float(loss(pert[0], per_view, sub) - loss(pert[1], per_view, sub)) / (2 * h)
         g = float(params[which].grad[j].reshape(-1)[comp])
         ref = float(params[which].grad.abs().max())
         err = abs(g - fd) / (abs(fd) + 1e-3 * ref + 1e-9)
         worst = max(worst, err)
-        assert err < 1e-3, (PARAMS[which], j, comp, g, fd)
+        assert err < 1e-3, (PARAMS[which], j, comp, g, fd, n_rays)
         checked += 1
+        many_rays += int(n_rays > 64)
     assert checked == 64, checked
+    assert many_rays >= 8, many_rays
     print(f"config 3: capacities stored {stored} / lists {lists}, max composited {ncomp}; "
-          f"64 parameters, worst relative error {worst:.2e}")
+          f"64 parameters ({many_rays} seen by more than 64 rays), worst relative error {worst:.2e}")

@@ -2,7 +2,8 @@
 
 Bars: hit sets/order/distances bit-exact (same snapshot and rays); image
 within 1e-4 max-abs (1e-5 with the GPU's own snapshot and rays); gradients
-within 1e-3 relative to the largest component."""
+within 1e-3 relative element by element (|a - b| <= 1e-3 |b| + 1e-6 max|b|)
+and to the largest component."""
 import numpy as np
 import pytest
 import torch
@@ -10,7 +11,7 @@ import torch
 from gray_amd import scenes
 from oracle import oracle as O
 
-from gpu_util import mock_camera, rays_of, rel_err, setup_native
+from gpu_util import assert_grads_close, comparable_gaussians, mock_camera, rays_of, rel_err, setup_native
 
 pytestmark = pytest.mark.gpu
 
@@ -175,14 +176,13 @@ def test_backward_matches_oracle(tiny):
     vis = g.was_visible.reshape(-1).cpu().numpy().astype(np.uint8)
     gdc, grest, gmean = O.sh_backward(scene.mean, scene.sh_rest, cam.origin.astype(np.float32), rgb, ob["rgb"],
                                       vis, 3, grad_mean=ob["mean"])
-    assert rel_err(g.grad_channels.cpu(), ob["rgb"]) < 1e-3
-    assert rel_err(g.grad_opacity.cpu(), ob["opacity"]) < 1e-3
-    assert rel_err(g.grad_scale.cpu(), ob["scale"]) < 1e-3
-    assert rel_err(g.grad_rotation.cpu(), ob["rotation"]) < 1e-3
-    assert rel_err(g.grad_mean.cpu(), gmean) < 1e-3
-    assert rel_err(g.grad_sh_coeffs_dc.cpu(), gdc) < 1e-3
-    assert rel_err(g.grad_sh_coeffs_rest.cpu(), grest) < 1e-3
-    assert rel_err(g.pruning_weight.cpu(), ob["pruning"]) < 1e-3
+    ok = comparable_gaussians(nr.hits.n_comp[: 64 * 64].cpu().numpy(), fwd, geo, ro, rd)
+    assert_grads_close([
+        ("channels", g.grad_channels.cpu(), ob["rgb"]), ("opacity", g.grad_opacity.cpu(), ob["opacity"]),
+        ("scale", g.grad_scale.cpu(), ob["scale"]), ("rotation", g.grad_rotation.cpu(), ob["rotation"]),
+        ("mean", g.grad_mean.cpu(), gmean), ("sh_dc", g.grad_sh_coeffs_dc.cpu(), gdc),
+        ("sh_rest", g.grad_sh_coeffs_rest.cpu(), grest), ("pruning", g.pruning_weight.cpu(), ob["pruning"]),
+    ], ok)
     assert np.array_equal(vis > 0, (np.abs(ob["opacity"].reshape(-1)) > 0) | (ob["pruning"].reshape(-1) > 0))
 
 

@@ -273,6 +273,41 @@ def test_random_scene_against_autograd(oracle_lib, seed, tthr, p):
         assert np.abs(a - b).max() < 1e-3 * scale + 1e-7, (key, np.abs(a - b).max(), scale)
 
 
+@pytest.mark.parametrize("tthr", [0.0, 0.3])
+def test_fixed_structure_render_equals_free_render(tthr):
+    """torch_ref.render with the hit structure given (composited order and
+    skipped hits) equals the render that decides it, value and gradient, so
+    finite differences with the structure held fixed differentiate the same
+    function within a piece."""
+    mean, q, s, logit, colors = random_small(5)
+    ro, rd = O.pinhole_rays(12, 12, 1.0, np.array([0, 0, 2.5], np.float32), np.eye(3))
+    ro, rd = torch.tensor(ro, dtype=torch.float64), torch.tensor(rd, dtype=torch.float64)
+    t = lambda a: torch.tensor(np.asarray(a, np.float64), requires_grad=True)  # noqa: E731
+    outs = []
+    free = None
+    for fixed_mode in (False, True):
+        tm, ts, tc, tq, to = t(mean), t(s), t(colors), t(q), t(logit)
+        fixed = None
+        if fixed_mode:
+            hits = torch_ref.hit_mask(tm, tq, ts, to, ro, rd)
+            fixed = []
+            for r in range(ro.shape[0]):
+                comp = free[1]["order"][r]
+                skipped = [int(g) for g in torch.nonzero(hits[r]).reshape(-1) if int(g) not in comp]
+                fixed.append((comp, skipped))
+        img, extra = torch_ref.render(tm, tq, ts, to, tc, ro, rd, 2.0, 0.01, tthr, (0.2, 0.3, 0.1), fixed=fixed)
+        if not fixed_mode:
+            free = (img, extra)
+        gout = torch.tensor(np.random.default_rng(1).normal(size=img.shape))
+        (img * gout).sum().backward()
+        outs.append((img.detach(), [x.grad.clone() for x in (tm, ts, tc, tq, to)]))
+    assert sum(len(o) for o in free[1]["order"]) > 50
+    # equal up to the order of the (float64) truncation-estimate sums
+    assert torch.allclose(outs[0][0], outs[1][0], rtol=1e-12, atol=1e-15)
+    for a, b in zip(outs[0][1], outs[1][1]):
+        assert torch.allclose(a, b, rtol=1e-12, atol=1e-15)
+
+
 def test_invariants(oracle_lib):
     mean, q, s, logit, colors = random_small(11, n=96)
     ro, rd = O.pinhole_rays(16, 16, 1.0, np.array([0, 0, 2.5], np.float32), np.eye(3))
