@@ -8,7 +8,13 @@ Prints one JSON line (metric/value/unit/...) with the dominant kernel's HBM
 roofline (HIP-event launch time, HBM traffic from rocprofv3 counters) and a
 bounded CPU-oracle baseline timed on the host cores.
 
-  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-traffic]
+  python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-traffic] [--no-extra]
+
+The default line also carries a "render" block (3M Gaussians, 1600x1063, 120
+poses: frames/s batched and one camera per launch, BVH build ms) and a
+"serving" block (2M Gaussians, Poisson requests of mixed resolution at 0.5x,
+1x and 2x the single-stream capacity: p50/p99 and frames/s batched against
+one launch per request), measured in the same run after the training step.
 
 Other workloads (--workload): render (3M Gaussians, 1600x1063, 120 poses),
 fwdbwd (200K Gaussians, 8 cameras at 512x512 in one launch), serve (2M
@@ -612,10 +618,17 @@ def run_serve(args):
         "ms_per_step": 1e3 / top["frames_per_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic (seeded surface scene, seeded Poisson requests)",
         "config": {"workload": "serve_2M_mixed_poisson", "gaussians": n, "slots": args.serve_slots,
+                   "scene_copies_in_hbm": 1,
                    "hbm_budget_gb": args.serve_budget_gb, "list_capacity": 256,
                    "resolutions": ["256x256", "512x384", "1280x720", "1600x900"]},
         "single_stream_capacity_per_s": capacity,
         "rates": rows,
+        "summary": {
+            "mean_batch_2x": rows[-1]["batched"]["mean_batch"],
+            "p99_ms_0.5x_batched_vs_plain": [rows[0]["batched"]["p99_ms"], rows[0]["plain"]["p99_ms"]],
+            "frames_per_s_2x_batched_vs_plain": [rows[-1]["batched"]["frames_per_s"],
+                                                 rows[-1]["plain"]["frames_per_s"]],
+        },
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -627,8 +640,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="train", choices=["train", "render", "fwdbwd", "serve", "tiny"])
-    ap.add_argument("--serve-requests", type=int, default=120)
-    ap.add_argument("--serve-slots", type=int, default=3)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-budget-gb", type=float, default=48.0)
     ap.add_argument("--gaussians", type=int, default=None, help="default: the workload's own size")
     ap.add_argument("--render-list-gb", type=float, default=48.0, help="render: hit-list memory of a launch")
@@ -637,6 +650,8 @@ def main():
     ap.add_argument("--loss-steps", type=int, default=20, help="train: further steps with the L1 + SSIM loss")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="train: skip the render-only and serving blocks of the default line")
     ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -662,6 +677,18 @@ def main():
 
     torch.cuda.set_device(0)
     out = run_train(args, traffic)
+    if not args.no_extra:
+        # the metric's other half on the same GPU in the same run: render-only
+        # frames/s (3M Gaussians) and served p50/p99 against plain launches
+        # (2M Gaussians), each at its own workload size
+        args.gaussians = None
+        for key, fn in (("render", run_render), ("serving", run_serve)):
+            torch.cuda.empty_cache()
+            t0 = time.perf_counter()
+            block = fn(args)
+            block["wall_s"] = round(time.perf_counter() - t0, 1)
+            out[key] = {k: v for k, v in block.items()
+                        if k not in ("n_gpus", "higher_is_better", "scaling", "vs_baseline", "cpu_baseline")}
     print(json.dumps(out), flush=True)
 
 

@@ -113,6 +113,7 @@ def _load():
     sig = {
         "gray_ctx_create": (i32, [C.POINTER(vp)]),
         "gray_ctx_destroy": (None, [vp]),
+        "gray_ctx_share_scene": (i32, [vp, vp]),
         "gray_last_error": (C.c_char_p, []),
         "gray_abi_version": (i32, []),
         "gray_device_status": (i32, [vp, vp]),
@@ -147,7 +148,7 @@ def _load():
     return lib
 
 
-ABI_VERSION = 10  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
+ABI_VERSION = 11  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
 LIB = _load()
 if LIB.gray_abi_version() != ABI_VERSION:
     raise ImportError(f"libgray_hip.so has ABI {LIB.gray_abi_version()}, the binding expects {ABI_VERSION}; "
@@ -158,7 +159,7 @@ EXPORTED = (
     "gray_sh_backward", "gray_render_forward", "gray_render_backward", "gray_adam_step", "gray_debug_hitlists",
     "gray_debug_snapshot", "gray_debug_bvh", "gray_debug_list_counts", "gray_render_stage_ms",
     "gray_render_reserve", "gray_loss_workspace_bytes", "gray_loss_forward", "gray_loss_backward",
-    "gray_render_ellipsoids", "gray_debug_wide", "gray_frame_setup", "gray_loss_noalloc",
+    "gray_render_ellipsoids", "gray_debug_wide", "gray_frame_setup", "gray_loss_noalloc", "gray_ctx_share_scene",
 )
 LOSS_NOALLOC_MAX_BYTES = 80 << 20  # GRAY_LOSS_NOALLOC_MAX_BYTES
 

@@ -831,6 +831,10 @@ extern "C" int gray_bvh_update(gray_ctx *ctx, const gray_params *params, const g
         gray_set_error(GRAY_ERR_INVALID, "gray_bvh_update: rotation must be 16-byte aligned (one float4 per Gaussian)");
         return GRAY_ERR_INVALID;
     }
+    if (ctx->scene) {
+        gray_set_error(GRAY_ERR_STATE, "gray_bvh_update: the context renders a shared scene; update the scene's context");
+        return GRAY_ERR_STATE;
+    }
     hipStream_t s = (hipStream_t)stream;
     if (!rebuild && (!ctx->built || ctx->n != n)) {
         gray_set_error(GRAY_ERR_STATE, "gray_bvh_update: refit needs a prior rebuild with the same count");
@@ -877,16 +881,18 @@ extern "C" int gray_bvh_set_timing(gray_ctx *ctx, int enable) {
 
 extern "C" float gray_bvh_last_build_ms(gray_ctx *ctx) { return ctx ? ctx->last_build_ms : -1.0f; }
 
-extern "C" int gray_bvh_info(gray_ctx *ctx, int64_t *num_gaussians, int64_t *num_nodes, int64_t *depth_bound) {
-    if (!ctx) return GRAY_ERR_INVALID;
+extern "C" int gray_bvh_info(gray_ctx *ctx_in, int64_t *num_gaussians, int64_t *num_nodes, int64_t *depth_bound) {
+    if (!ctx_in) return GRAY_ERR_INVALID;
+    const gray_ctx *ctx = scene_of(ctx_in);
     if (num_gaussians) *num_gaussians = ctx->n;
     if (num_nodes) *num_nodes = ctx->n > 1 ? ctx->n - 1 : (ctx->n == 1 ? 1 : 0);
     if (depth_bound) *depth_bound = GRAY_STACK;
     return GRAY_OK;
 }
 
-extern "C" int gray_debug_bvh(gray_ctx *ctx, float *nodes, uint32_t *sorted_ids, uint32_t *keys, void *stream) {
-    if (!ctx) return GRAY_ERR_INVALID;
+extern "C" int gray_debug_bvh(gray_ctx *ctx_in, float *nodes, uint32_t *sorted_ids, uint32_t *keys, void *stream) {
+    if (!ctx_in) return GRAY_ERR_INVALID;
+    const gray_ctx *ctx = scene_of(ctx_in);
     hipStream_t s = (hipStream_t)stream;
     const int64_t n = ctx->n;
     if (n == 0) return GRAY_OK;
@@ -897,8 +903,9 @@ extern "C" int gray_debug_bvh(gray_ctx *ctx, float *nodes, uint32_t *sorted_ids,
     return GRAY_OK;
 }
 
-extern "C" int gray_debug_wide(gray_ctx *ctx, uint32_t *nodes, uint8_t *flags, void *stream) {
-    if (!ctx) return GRAY_ERR_INVALID;
+extern "C" int gray_debug_wide(gray_ctx *ctx_in, uint32_t *nodes, uint8_t *flags, void *stream) {
+    if (!ctx_in) return GRAY_ERR_INVALID;
+    const gray_ctx *ctx = scene_of(ctx_in);
     hipStream_t s = (hipStream_t)stream;
     const int64_t n = ctx->n;
     if (n == 0) return GRAY_OK;
@@ -908,8 +915,9 @@ extern "C" int gray_debug_wide(gray_ctx *ctx, uint32_t *nodes, uint8_t *flags, v
     return GRAY_OK;
 }
 
-extern "C" int gray_debug_snapshot(gray_ctx *ctx, float *geo, float *aabb, void *stream) {
-    if (!ctx) return GRAY_ERR_INVALID;
+extern "C" int gray_debug_snapshot(gray_ctx *ctx_in, float *geo, float *aabb, void *stream) {
+    if (!ctx_in) return GRAY_ERR_INVALID;
+    const gray_ctx *ctx = scene_of(ctx_in);
     hipStream_t s = (hipStream_t)stream;
     if (ctx->n == 0) return GRAY_OK;
     if (geo) GRAY_HIP(hipMemcpyAsync(geo, ctx->geo, sizeof(GeoRecord) * ctx->n, hipMemcpyDeviceToDevice, s));

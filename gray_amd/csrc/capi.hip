@@ -68,6 +68,19 @@ extern "C" void gray_ctx_destroy(gray_ctx *ctx) {
     delete ctx;
 }
 
+extern "C" int gray_ctx_share_scene(gray_ctx *ctx, gray_ctx *scene) {
+    if (!ctx || ctx == scene) {
+        gray_set_error(GRAY_ERR_INVALID, "ctx_share_scene: null context or a context sharing itself");
+        return GRAY_ERR_INVALID;
+    }
+    if (scene && (scene->scene || scene->device != ctx->device)) {
+        gray_set_error(GRAY_ERR_INVALID, "ctx_share_scene: the scene must own its BVH and live on the same device");
+        return GRAY_ERR_INVALID;
+    }
+    ctx->scene = scene;
+    return GRAY_OK;
+}
+
 // One thread writes the camera, flag and counter fields; every thread clears
 // 16 bytes of was_visible (bytewise at an unaligned base or the tail).
 __global__ void k_frame_setup(gray_frame_args a, int aligned16) {

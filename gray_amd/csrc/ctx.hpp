@@ -7,6 +7,9 @@
 
 struct gray_ctx {
     int device = 0;
+    // when set (gray_ctx_share_scene), renders read this context's geometry
+    // snapshot and BVH; the fields below then keep only the render scratch
+    gray_ctx *scene = nullptr;
     int64_t n = 0;         // Gaussians in the current snapshot
     int64_t cap = 0;       // allocated capacity (Gaussians)
     bool built = false;    // a topology exists for n
@@ -60,6 +63,9 @@ struct gray_ctx {
     hipEvent_t fev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     bool fev_recorded = false;
 };
+
+// The context whose snapshot and BVH a render of ctx reads.
+inline const gray_ctx *scene_of(const gray_ctx *ctx) { return ctx->scene ? ctx->scene : ctx; }
 
 int gray_ctx_reserve(gray_ctx *ctx, int64_t n);
 int gray_refit(gray_ctx *ctx, hipStream_t s);

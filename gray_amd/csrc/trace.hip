@@ -1950,6 +1950,7 @@ __global__ void k_debug_sort(int rays, int maxh, uint32_t *__restrict__ ids, flo
 
 int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int nviews, TraceArgs &A) {
     memset(&A, 0, sizeof(A));
+    const gray_ctx *S = scene_of(ctx);
     const gray_knobs &k = args->knobs;
     if (!k.exp_power || !k.alpha_threshold || !k.t_threshold || !k.eps_forward_normalization || !k.eps_scale_grad ||
         !k.background_channels || !k.render_depth) {
@@ -1970,7 +1971,7 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
         const bool pinhole_possible = !explicit_rays || v.rays_enable != nullptr;
         if ((explicit_rays && !v.rays_origin) ||
             (pinhole_possible && (!v.origin || !v.rotation_c2w || !v.vertical_fov)) || !v.znear || !v.zfar ||
-            !v.out_channels || (ctx->n > 0 && !v.rgb)) {
+            !v.out_channels || (S->n > 0 && !v.rgb)) {
             gray_set_error(GRAY_ERR_INVALID, "render: view %d lacks a camera, rgb or output pointer", first_view + i);
             return GRAY_ERR_INVALID;
         }
@@ -2015,12 +2016,12 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
     A.fp16 = args->fp16_hits ? 1 : 0;
     A.capacity = args->hit_capacity < 0 ? 0 : args->hit_capacity;
     A.use_bvh = args->use_bvh;
-    A.num_gaussians = (int)ctx->n;
+    A.num_gaussians = (int)S->n;
     A.tile_stats = args->tile_stats;
     A.tile_base_launch = 0;
-    A.geo = ctx->geo;
-    A.nodes = ctx->nodes;
-    A.l2w = ctx->l2w;
+    A.geo = S->geo;
+    A.nodes = S->nodes;
+    A.l2w = S->l2w;
     A.error_flag = ctx->err;
     A.heavy_count = nullptr;
     A.heavy_rays = nullptr;
@@ -2090,7 +2091,7 @@ int check_render(gray_ctx *ctx, const gray_render_args *args) {
         gray_set_error(GRAY_ERR_INVALID, "render: null context/arguments or no views");
         return GRAY_ERR_INVALID;
     }
-    if (!ctx->built) {
+    if (!scene_of(ctx)->built) {
         gray_set_error(GRAY_ERR_STATE, "render: no BVH (call gray_bvh_update with rebuild=1 first)");
         return GRAY_ERR_STATE;
     }
@@ -2147,7 +2148,7 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             A.task_count_out = count;
             A.budget = args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
             if ((rc = mark(0))) return rc;
-            const Bvh4Node *cnodes = ctx->wide;
+            const Bvh4Node *cnodes = scene_of(ctx)->wide;
             hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, cnodes);
             GRAY_HIP(hipGetLastError());
             if ((rc = mark(1))) return rc;
@@ -2242,12 +2243,13 @@ extern "C" int gray_render_backward(gray_ctx *ctx, const gray_render_args *args,
         gray_set_error(GRAY_ERR_INVALID, "render_backward: null params or gradient pointers");
         return GRAY_ERR_INVALID;
     }
-    if (params->count != ctx->n) {
+    const int64_t n_scene = scene_of(ctx)->n;
+    if (params->count != n_scene) {
         gray_set_error(GRAY_ERR_STATE, "render_backward: %lld Gaussians but the snapshot has %lld",
-                       (long long)params->count, (long long)ctx->n);
+                       (long long)params->count, (long long)n_scene);
         return GRAY_ERR_STATE;
     }
-    if (ctx->n == 0) return GRAY_OK;
+    if (n_scene == 0) return GRAY_OK;
     hipStream_t s = (hipStream_t)stream;
     for (int first = 0; first < args->num_views; first += GRAY_MAX_VIEWS) {
         const int nv = std::min(GRAY_MAX_VIEWS, args->num_views - first);

@@ -1,9 +1,11 @@
 """Serving loop: one scene resident in HBM, camera requests of mixed
 resolution rendered with continuous batching.
 
-`SceneServer` keeps the Gaussians on the device and owns `slots` execution
-slots.  Each slot is a library context (its own geometry snapshot, BVH and
-hit-list buffers, reserved up front) and its own stream.  The host loop:
+`SceneServer` keeps the Gaussians on the device, with one geometry snapshot
+and one BVH (the scene's library context), and owns `slots` execution
+slots.  Each slot is a library context that renders the shared scene
+(gray_ctx_share_scene) with its own hit-list and task buffers, reserved up
+front, and its own stream.  The host loop:
   * moves arrived requests to a FIFO queue;
   * retires slots whose batch finished (HIP event query) and stamps their
     requests with the device completion time;
@@ -88,6 +90,8 @@ class Request:
 class _Slot:
     def __init__(self, server: "SceneServer", max_lanes: int):
         self.ctx = L.Context()
+        # renders read the server's one snapshot and BVH; the slot owns only scratch
+        L.check(L.LIB.gray_ctx_share_scene(self.ctx.handle, server.scene.handle))
         self.stream = torch.cuda.Stream()
         self.max_lanes = max_lanes
         self.batch: List[Request] = []
@@ -97,17 +101,15 @@ class _Slot:
             self.cams = torch.zeros(MAX_VIEWS, CAM_FLOATS, dtype=F32, device=server.device)
             self.rgb = torch.zeros(MAX_VIEWS, server.n, 3, dtype=F32, device=server.device)
         self.cams_host = torch.zeros(MAX_VIEWS, CAM_FLOATS, dtype=F32, pin_memory=True)
-        sp = L.stream_ptr(self.stream)
-        L.check(L.LIB.gray_bvh_update(self.ctx.handle, C.byref(server.params), C.byref(server.knobs()), 1, sp))
         if server.list_capacity > 0:
             L.check(L.LIB.gray_render_reserve(self.ctx.handle, max_lanes, server.list_capacity))
         self.stream.synchronize()
-        L.check(L.LIB.gray_device_status(self.ctx.handle, sp))
+        L.check(L.LIB.gray_device_status(self.ctx.handle, L.stream_ptr(self.stream)))
 
 
 class SceneServer:
     def __init__(self, mean, rotation, scale, opacity, sh_dc, sh_rest, settings: Optional[RenderSettings] = None,
-                 slots: int = 3, budget_bytes: int = 32 << 30, list_capacity: int = 256, device=None):
+                 slots: int = 2, budget_bytes: int = 32 << 30, list_capacity: int = 256, device=None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         f = lambda t: t.detach().to(self.device, F32).contiguous()  # noqa: E731
         self.tensors = [f(t) for t in (mean, rotation, scale, opacity, sh_dc, sh_rest)]
@@ -130,6 +132,9 @@ class SceneServer:
         per_lane = self.list_capacity * 16 + TASK_BYTES_PER_LANE
         self.slot_lanes = int(budget_bytes // max(1, slots) // per_lane)
         self.budget_bytes = budget_bytes
+        # the one resident snapshot + BVH of the scene
+        self.scene = L.Context()
+        self._rebuild()
         self.slots = [_Slot(self, self.slot_lanes) for _ in range(slots)]
         self._queue: List[Request] = []
         self._h0 = time.perf_counter()
@@ -137,20 +142,27 @@ class SceneServer:
     def knobs(self, depth: bool = False) -> L.Knobs:
         return L.Knobs(*(k.data_ptr() for k in (self._k_depth if depth else self._k)))
 
+    def _rebuild(self) -> None:
+        """Snapshot + BVH of the scene, on the current stream, finished before
+        any slot renders it."""
+        stream = torch.cuda.current_stream(self.device)
+        L.check(L.LIB.gray_bvh_update(self.scene.handle, C.byref(self.params), C.byref(self.knobs()), 1,
+                                      L.stream_ptr(stream)))
+        stream.synchronize()
+        self.scene.status(stream)
+
     def set_global_scale_factor(self, value: float) -> None:
         """The viewer's scaling modifier (view.py:171-176, 407-410): waits for
         the batches in flight (they keep their images, made with the old
-        factor; poll() still returns them) and rebuilds every slot's BVH."""
+        factor; poll() still returns them) and rebuilds the scene's BVH, which
+        every slot renders."""
         if float(self._k[3].item()) == float(value):
             return
         for sl in self.slots:
             sl.stream.synchronize()
         self._k[3].fill_(float(value))
         self._k_depth[3].fill_(float(value))
-        torch.cuda.current_stream(self.device).synchronize()
-        for sl in self.slots:
-            sp = L.stream_ptr(sl.stream)
-            L.check(L.LIB.gray_bvh_update(sl.ctx.handle, C.byref(self.params), C.byref(self.knobs()), 1, sp))
+        self._rebuild()
 
     # ---- one batch ---------------------------------------------------------
     def _launch(self, slot: _Slot, reqs: Sequence[Request]) -> None:

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 10
+#define GRAY_ABI_VERSION 11
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -208,6 +208,17 @@ typedef struct {
  * Creates the BVH holder on the current HIP device. */
 int gray_ctx_create(gray_ctx **out);
 void gray_ctx_destroy(gray_ctx *ctx);
+
+/* No reference counterpart (serving: one resident scene, several launches in
+ * flight).  Makes renders of `ctx` (forward, backward, ellipsoids, debug
+ * copies) read the geometry snapshot and BVH of `scene` instead of its own;
+ * ctx keeps only its render scratch (hit lists, task buffers, error flag,
+ * stage events), so any number of contexts render one scene held once in
+ * HBM, each on its own stream.  gray_bvh_update on ctx is then refused: the
+ * scene's context updates the BVH, and the caller orders that update against
+ * the renders in flight.  `scene` must own its BVH, live on the same device
+ * and outlive the sharing; NULL detaches. */
+int gray_ctx_share_scene(gray_ctx *ctx, gray_ctx *scene);
 const char *gray_last_error(void);
 int gray_abi_version(void);
 

@@ -1,6 +1,7 @@
 """Serving loop on the GPU: every request renders the same alone and inside
 mixed-resolution batches, in continuous batching and in plain one-launch-per-
 request mode, and the device result equals the single-camera drop-in path."""
+import ctypes as C
 import math
 
 import numpy as np
@@ -150,3 +151,37 @@ def test_submit_poll(served):
     server.submit(big)
     out = server.poll()
     assert out == [big] and "exceeds" in big.error
+
+
+def test_slots_share_one_scene():
+    """Every slot renders the server's one snapshot and BVH: a further slot
+    adds only its render scratch to device memory, far less than the scene's
+    geometry, and it sees that geometry."""
+    from gray_amd import _lib as L
+    from gray_amd.serving import _Slot
+
+    scene = scenes.surface_scene(300_000, 20, -4.5, 0.4, seed=2)
+    t = [getattr(scene, k).cuda() for k in ("mean", "rotation", "scale", "opacity", "sh_dc", "sh_rest")]
+    server = SceneServer(*t, settings=RenderSettings(t_threshold=0.03), slots=1, budget_bytes=1 << 30,
+                         list_capacity=0)
+    torch.cuda.synchronize()
+    free0, reserved0 = torch.cuda.mem_get_info()[0], torch.cuda.memory_reserved()
+    extra = _Slot(server, server.slot_lanes)
+    torch.cuda.synchronize()
+    free1, reserved1 = torch.cuda.mem_get_info()[0], torch.cuda.memory_reserved()
+    library_bytes = (free0 - free1) - (reserved1 - reserved0)  # outside torch's allocator
+    scene_bytes = 300_000 * (64 + 48 + 24 + 64 + 64)  # snapshot, L2W, AABB, binary and 4-wide nodes
+    assert library_bytes < scene_bytes // 4, (library_bytes, scene_bytes)
+    n = C.c_int64()
+    L.check(L.LIB.gray_bvh_info(extra.ctx.handle, C.byref(n), None, None))
+    assert n.value == 300_000
+    # the slot's geometry is the scene's, and a slot cannot rebuild it
+    geo_scene = torch.zeros(300_000, 16, device="cuda")
+    geo_slot = torch.zeros(300_000, 16, device="cuda")
+    L.check(L.LIB.gray_debug_snapshot(server.scene.handle, geo_scene.data_ptr(), None, L.stream_ptr()))
+    L.check(L.LIB.gray_debug_snapshot(extra.ctx.handle, geo_slot.data_ptr(), None, L.stream_ptr()))
+    torch.cuda.synchronize()
+    assert torch.equal(geo_scene, geo_slot)
+    rc = L.LIB.gray_bvh_update(extra.ctx.handle, C.byref(server.params), C.byref(server.knobs()), 1,
+                               L.stream_ptr())
+    assert rc == L.GRAY_ERR_STATE
