@@ -40,6 +40,7 @@ namespace {
 
 struct LaneRay {
     float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+    float one;  // 1.0f, opaque to the compiler (set by make_ray): keeps half_minus a v_fma_mix_f32
 };
 
 __device__ __forceinline__ bool slab_hit(const LaneRay &r, float x0, float y0, float z0, float x1, float y1, float z1,
@@ -121,6 +122,9 @@ __device__ __forceinline__ bool make_ray(const DevKnobs &kn, const DevView &V, i
     r.ix = 1.0f / dx;
     r.iy = 1.0f / dy;
     r.iz = 1.0f / dz;
+    float one = 1.0f;
+    asm volatile("" : "+v"(one));
+    r.one = one;
     const float l2 = (dx * dx + dy * dy) + dz * dz;
     return l2 != 0.0f;
 }
@@ -168,18 +172,35 @@ __device__ __forceinline__ Node4V load_node4(const Bvh4Node *__restrict__ wide, 
     return n;
 }
 
-__device__ __forceinline__ float half_lo(uint32_t v) { return __half2float(__ushort_as_half((unsigned short)(v & 0xffffu))); }
-__device__ __forceinline__ float half_hi(uint32_t v) { return __half2float(__ushort_as_half((unsigned short)(v >> 16))); }
+// Half-precision box bound (low or high half of a word) minus a ray origin
+// coordinate: the conversion is exact, the difference rounded once; one
+// v_fma_mix_f32 (half operand selected by op_sel) per bound.
+template <bool kHigh>
+__device__ __forceinline__ float half_minus(uint32_t v, float o, float one) {
+    const _Float16 h = __builtin_bit_cast(_Float16, (unsigned short)(kHigh ? (v >> 16) : (v & 0xffffu)));
+    return __builtin_fmaf((float)h, one, -o);
+}
 
 // Child k's box test; the box words of child k are word k/2 of each pair, its
-// half is k%2.
+// half is k%2.  The same slab test as slab_hit (differences to the origin
+// times the inverse direction; relative slack, so the cull stays conservative
+// under rounding at any distance), written without branches: the slack is
+// 1e-5 |tf| + 1e-7 on the exit parameter, and an infinite tf gives an
+// infinite (tf = +inf) or NaN (tf = -inf: no hit) slackened exit.
 template <int k>
 __device__ __forceinline__ bool child_hit(const LaneRay &r, const Node4V &n) {
     constexpr bool h = (k & 1) != 0;
     const uint32_t lx = k < 2 ? n.a.x : n.a.y, ly = k < 2 ? n.a.z : n.a.w, lz = k < 2 ? n.b.x : n.b.y;
     const uint32_t hx = k < 2 ? n.b.z : n.b.w, hy = k < 2 ? n.c.x : n.c.y, hz = k < 2 ? n.c.z : n.c.w;
-    auto f = [](uint32_t v) { return h ? half_hi(v) : half_lo(v); };
-    return slab_hit(r, f(lx), f(ly), f(lz), f(hx), f(hy), f(hz), 0.0f);
+    const float ax0 = half_minus<h>(lx, r.ox, r.one), ax1 = half_minus<h>(hx, r.ox, r.one);
+    const float tx0 = ax0 * r.ix, tx1 = ax1 * r.ix;
+    const float ty0 = half_minus<h>(ly, r.oy, r.one) * r.iy, ty1 = half_minus<h>(hy, r.oy, r.one) * r.iy;
+    const float tz0 = half_minus<h>(lz, r.oz, r.one) * r.iz, tz1 = half_minus<h>(hz, r.oz, r.one) * r.iz;
+    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
+    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    const float tfs = __builtin_fmaf(fabsf(tf), 1e-5f, tf) + 1e-7f;
+    // an empty box (min above max, e.g. a subtree of invisible Gaussians) is no hit
+    return (ax0 <= ax1) & (tn <= tfs) & (tfs >= 0.0f);
 }
 
 __device__ __forceinline__ GeoV load_geo(const GeoRecord *__restrict__ geo, uint32_t gid) {
@@ -544,11 +565,12 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
         if (h != GRAY_EMPTY && qn <= kLeafQueue - 4) {
             moved = true;
             ++visits;
+            // an unused child (ref GRAY_EMPTY) stays GRAY_EMPTY whatever its box gives
             uint32_t ch[4];
-            ch[0] = (nd.ch.x != GRAY_EMPTY && child_hit<0>(r, nd)) ? nd.ch.x : GRAY_EMPTY;
-            ch[1] = (nd.ch.y != GRAY_EMPTY && child_hit<1>(r, nd)) ? nd.ch.y : GRAY_EMPTY;
-            ch[2] = (nd.ch.z != GRAY_EMPTY && child_hit<2>(r, nd)) ? nd.ch.z : GRAY_EMPTY;
-            ch[3] = (nd.ch.w != GRAY_EMPTY && child_hit<3>(r, nd)) ? nd.ch.w : GRAY_EMPTY;
+            ch[0] = child_hit<0>(r, nd) ? nd.ch.x : GRAY_EMPTY;
+            ch[1] = child_hit<1>(r, nd) ? nd.ch.y : GRAY_EMPTY;
+            ch[2] = child_hit<2>(r, nd) ? nd.ch.z : GRAY_EMPTY;
+            ch[3] = child_hit<3>(r, nd) ? nd.ch.w : GRAY_EMPTY;
             nh = GRAY_EMPTY;
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
