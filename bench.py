@@ -229,18 +229,26 @@ def measure_traffic(args):
 
 
 def copy_peak_gbs():
-    """Measured device-to-device copy rate (read + write bytes / time)."""
+    """Measured device-to-device copy rate (read + write bytes / time) of the
+    library's non-temporal 16-byte streaming copy kernel over 1 GiB."""
     import torch
+
+    from gray_amd import _lib as L
 
     n = 256 * 1024 * 1024
     a = torch.empty(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
-    b.copy_(a)
+    sp = L.stream_ptr()
+
+    def copy():
+        L.check(L.LIB.gray_debug_stream_copy(b.data_ptr(), a.data_ptr(), n * 4, sp))
+
+    copy()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(10):
-        b.copy_(a)
+        copy()
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / 10

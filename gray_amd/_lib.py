@@ -140,6 +140,7 @@ def _load():
         "gray_loss_forward": (i32, [C.POINTER(LossArgs), vp]),
         "gray_loss_backward": (i32, [C.POINTER(LossArgs), vp]),
         "gray_loss_noalloc": (i32, [C.POINTER(LossArgs), vp]),
+        "gray_debug_stream_copy": (i32, [vp, vp, i64, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -160,6 +161,7 @@ EXPORTED = (
     "gray_debug_snapshot", "gray_debug_bvh", "gray_debug_list_counts", "gray_render_stage_ms",
     "gray_render_reserve", "gray_loss_workspace_bytes", "gray_loss_forward", "gray_loss_backward",
     "gray_render_ellipsoids", "gray_debug_wide", "gray_frame_setup", "gray_loss_noalloc", "gray_ctx_share_scene",
+    "gray_debug_stream_copy",
 )
 LOSS_NOALLOC_MAX_BYTES = 80 << 20  # GRAY_LOSS_NOALLOC_MAX_BYTES
 

@@ -9,6 +9,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 #include "ctx.hpp"
 
 namespace {
@@ -190,6 +192,40 @@ extern "C" int gray_adam_step(gray_ctx *ctx, const gray_adam *adam, int32_t step
     A.d_step = adam->step_device;
     if (blocks == 0) return GRAY_OK;
     hipLaunchKernelGGL(k_adam, dim3(blocks), dim3(kThreads), 0, (hipStream_t)stream, A);
+    GRAY_HIP(hipGetLastError());
+    return GRAY_OK;
+}
+
+namespace {
+
+// Streaming copy of n16 16-byte vectors with non-temporal loads and stores,
+// grid-stride: the device's sustained read + write rate for a kernel that
+// moves each byte once (the reference point of the streaming kernels).
+__global__ __launch_bounds__(256) void k_stream_copy(const f4v *__restrict__ src, f4v *__restrict__ dst, int64_t n16) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += 4 * stride) {
+        f4v v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * stride < n16) v[k] = __builtin_nontemporal_load(src + i + k * stride);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k * stride < n16) __builtin_nontemporal_store(v[k], dst + i + k * stride);
+    }
+}
+
+}  // namespace
+
+extern "C" int gray_debug_stream_copy(void *dst, const void *src, int64_t bytes, void *stream) {
+    if (!dst || !src || bytes < 0 || (bytes & 15) || (((uintptr_t)dst | (uintptr_t)src) & 15)) {
+        gray_set_error(GRAY_ERR_INVALID, "stream_copy: null pointer, or size/alignment not a multiple of 16 bytes");
+        return GRAY_ERR_INVALID;
+    }
+    const int64_t n16 = bytes / 16;
+    if (n16 == 0) return GRAY_OK;
+    const int64_t blocks = std::min<int64_t>((n16 + 1023) / 1024, 256 * 64);
+    hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<const f4v *>(src), reinterpret_cast<f4v *>(dst), n16);
     GRAY_HIP(hipGetLastError());
     return GRAY_OK;
 }

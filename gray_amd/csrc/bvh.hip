@@ -53,26 +53,29 @@ __device__ __forceinline__ void box_empty(float *a) {
 // Snapshot: reference populate_bvh_kernel + create_transform_matrix
 // (bvh_wrapper.cu:8-53) and compute_scaling_factor (kernel.cu:3-22).
 // ---------------------------------------------------------------------------
+struct SnapF3 {
+    float x, y, z;
+};
+
 __global__ void k_snapshot(int n, const float *__restrict__ mean, const float *__restrict__ rotation,
                            const float *__restrict__ scale, const float *__restrict__ opacity, DevKnobs kn,
                            GeoRecord *__restrict__ geo, float *__restrict__ l2w, float *__restrict__ aabb) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const float athr = *kn.alpha_threshold, p = *kn.exp_power, gsf = *kn.gsf;
+    // mean and log-scale rows (12 bytes, 4-byte aligned) by one dwordx3 load each
+    const SnapF3 ls = reinterpret_cast<const SnapF3 *>(scale)[g];
+    const SnapF3 mv = reinterpret_cast<const SnapF3 *>(mean)[g];
     const float op = gray_sigmoid(opacity[g]);
     const float clip = gray_clip_factor(op, athr, p);
-    const float sx = (expf(scale[3 * g + 0]) * clip) * gsf;
-    const float sy = (expf(scale[3 * g + 1]) * clip) * gsf;
-    const float sz = (expf(scale[3 * g + 2]) * clip) * gsf;
+    const float sx = (expf(ls.x) * clip) * gsf;
+    const float sy = (expf(ls.y) * clip) * gsf;
+    const float sz = (expf(ls.z) * clip) * gsf;
     const bool visible = clip > 0.0f && (sx > 0.0f || sy > 0.0f || sz > 0.0f);
 
     const float4 q = reinterpret_cast<const float4 *>(rotation)[g];
-    float r = q.x, x = q.y, y = q.z, z = q.w;
-    const float norm = sqrtf(r * r + x * x + y * y + z * z);
-    r /= norm;
-    x /= norm;
-    y /= norm;
-    z /= norm;
+    const float inv_norm = 1.0f / sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    const float r = q.x * inv_norm, x = q.y * inv_norm, y = q.z * inv_norm, z = q.w * inv_norm;
     float R[9];
     R[0] = 1.f - 2.f * (y * y + z * z);
     R[1] = 2.f * (x * y - r * z);
@@ -96,7 +99,7 @@ __global__ void k_snapshot(int n, const float *__restrict__ mean, const float *_
             rec.m[3 * i + j] = R[3 * j + i] * inv[i];  // its inverse diag(1/s) R^T
         }
     }
-    const float mu[3] = {mean[3 * g + 0], mean[3 * g + 1], mean[3 * g + 2]};
+    const float mu[3] = {mv.x, mv.y, mv.z};
     rec.mu[0] = mu[0];
     rec.mu[1] = mu[1];
     rec.mu[2] = mu[2];

@@ -101,17 +101,68 @@ __device__ __forceinline__ int rest_count(int degree) { return (degree + 1) * (d
 // Bodies are instantiated per degree so that every coefficient and basis
 // array has compile-time indices and lives in registers; the kernels switch
 // once on the (launch-uniform) degree read from the device.
+// Three consecutive floats (4-byte aligned): one global_load/store_dwordx3.
+struct F3 {
+    float x, y, z;
+};
+// Four consecutive floats at 4-byte alignment: one global_load_dwordx4 (the
+// vector memory instructions need dword alignment only).
+typedef float f4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// The used coefficients of Gaussian g: fp32 in 16-byte vectors (a Gaussian's
+// K*3 floats are contiguous), so a wave issues 12 wide loads instead of 45
+// scalar ones over the same cache lines; fp16 element by element.
+template <int KC>
+__device__ __forceinline__ void load_rest(const float *__restrict__ rest, int g, int K, float (*c)[3]) {
+    const float *base = rest + (size_t)g * K * 3;
+    constexpr int NF = KC * 3, NV = NF / 4;
+    float f[NF > 0 ? NF : 1];
+    const f4a4 *p = reinterpret_cast<const f4a4 *>(base);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const f4a4 v = p[i];
+        f[4 * i] = v.x, f[4 * i + 1] = v.y, f[4 * i + 2] = v.z, f[4 * i + 3] = v.w;
+    }
+#pragma unroll
+    for (int i = 4 * NV; i < NF; ++i) f[i] = base[i];
+#pragma unroll
+    for (int k = 0; k < KC; ++k)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) c[k][ch] = f[3 * k + ch];
+}
+template <int KC>
+__device__ __forceinline__ void load_rest(const __half *__restrict__ rest, int g, int K, float (*c)[3]) {
+#pragma unroll
+    for (int k = 0; k < KC; ++k)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) c[k][ch] = ld(rest, ((size_t)g * K + k) * 3 + ch);
+}
+__device__ __forceinline__ F3 load3(const float *__restrict__ p, int g) { return reinterpret_cast<const F3 *>(p)[g]; }
+__device__ __forceinline__ F3 load3(const __half *__restrict__ p, int g) {
+    return F3{ld(p, 3 * (size_t)g), ld(p, 3 * (size_t)g + 1), ld(p, 3 * (size_t)g + 2)};
+}
+
+// Colour of Gaussian g for every view of the launch from its DC term, mean
+// and higher coefficients c.
+template <int DEG>
+__device__ __forceinline__ void sh_forward_eval(int g, const F3 &dcv, const F3 &mv, const float (*c)[3],
+                                                const ShViews &V);
+
 template <int DEG, typename T>
 __device__ __forceinline__ void sh_forward_body(int g, int K, const float *__restrict__ mean, const T *__restrict__ dc,
                                                 const T *__restrict__ rest, const ShViews &V) {
     constexpr int KC = DEG > 0 ? (DEG + 1) * (DEG + 1) - 1 : 0;
     float c[KC > 0 ? KC : 1][3];
-#pragma unroll
-    for (int k = 0; k < KC; ++k)
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) c[k][ch] = ld(rest, ((size_t)g * K + k) * 3 + ch);
-    const float d0 = ld(dc, 3 * (size_t)g), d1 = ld(dc, 3 * (size_t)g + 1), d2 = ld(dc, 3 * (size_t)g + 2);
-    const float mx = mean[3 * g], my = mean[3 * g + 1], mz = mean[3 * g + 2];
+    load_rest<KC>(rest, g, K, c);
+    sh_forward_eval<DEG>(g, load3(dc, g), load3(mean, g), c, V);
+}
+
+template <int DEG>
+__device__ __forceinline__ void sh_forward_eval(int g, const F3 &dcv, const F3 &mv, const float (*c)[3],
+                                                const ShViews &V) {
+    constexpr int KC = DEG > 0 ? (DEG + 1) * (DEG + 1) - 1 : 0;
+    const float d0 = dcv.x, d1 = dcv.y, d2 = dcv.z;
+    const float mx = mv.x, my = mv.y, mz = mv.z;
     for (int v = 0; v < V.num; ++v) {
         const float *o = V.origin[v];
         const float dx = mx - o[0], dy = my - o[1], dz = mz - o[2];
@@ -128,10 +179,7 @@ __device__ __forceinline__ void sh_forward_body(int g, int K, const float *__res
                 r2 = fmaf(b[k], c[k][2], r2);
             }
         }
-        float *out = V.rgb[v];
-        out[3 * g + 0] = fmaxf(r0 + 0.5f, 0.0f);
-        out[3 * g + 1] = fmaxf(r1 + 0.5f, 0.0f);
-        out[3 * g + 2] = fmaxf(r2 + 0.5f, 0.0f);
+        reinterpret_cast<F3 *>(V.rgb[v])[g] = F3{fmaxf(r0 + 0.5f, 0.0f), fmaxf(r1 + 0.5f, 0.0f), fmaxf(r2 + 0.5f, 0.0f)};
     }
 }
 
@@ -151,6 +199,75 @@ __global__ void k_sh_forward(int n, int K, const int32_t *__restrict__ cur_degre
         case 2: sh_forward_body<2>(g, K, mean, dc, rest, V); break;
         case 1: sh_forward_body<1>(g, K, mean, dc, rest, V); break;
         default: sh_forward_body<0>(g, K, mean, dc, rest, V); break;
+    }
+}
+
+// fp32 coefficients staged through LDS: each wave owns 64 consecutive
+// Gaussians, whose K*3*64 coefficient floats are one contiguous block, and
+// copies it into its LDS slice with global_load_lds_dwordx4 (every load
+// instruction moves 1 KB of consecutive memory, straight into LDS), then every
+// lane reads its own K*3 floats (stride K*3 dwords: odd for K = 15, so no
+// bank conflicts).  The last, partial wave of the array loads directly.
+constexpr int kShLdsWaves = 4;
+constexpr int kShMaxK = 15;
+
+__global__ __launch_bounds__(64 * kShLdsWaves) void k_sh_forward_lds(int n, int K, const int32_t *__restrict__ cur_degree,
+                                                                    const uint8_t *__restrict__ enable,
+                                                                    const float *__restrict__ mean,
+                                                                    const float *__restrict__ dc,
+                                                                    const float *__restrict__ rest, ShViews V) {
+    __shared__ float4 stage[kShLdsWaves][kShMaxK * 3 * 64 / 4];
+    if (!*enable) return;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g0 = (blockIdx.x * kShLdsWaves + wave) * 64;
+    if (g0 >= n) return;
+    const int g = g0 + lane;
+    const int deg = sh_degree(cur_degree, K);
+    if (g0 + 64 > n || deg == 0) {
+        if (g >= n) return;
+        switch (deg) {
+            case 3: sh_forward_body<3>(g, K, mean, dc, rest, V); break;
+            case 2: sh_forward_body<2>(g, K, mean, dc, rest, V); break;
+            case 1: sh_forward_body<1>(g, K, mean, dc, rest, V); break;
+            default: sh_forward_body<0>(g, K, mean, dc, rest, V); break;
+        }
+        return;
+    }
+    const F3 dcv = load3(dc, g), mv = load3(mean, g);  // in flight with the staging
+    // the wave's block: 64 * K * 3 floats = 48 K chunks of 16 bytes
+    const float4 *src = reinterpret_cast<const float4 *>(rest + (size_t)g0 * K * 3);
+    const int chunks = 48 * K;
+    for (int i = 0; i * 64 < chunks; ++i) {
+        const int c = i * 64 + lane;
+        if (c < chunks)
+            __builtin_amdgcn_global_load_lds(src + c, (__attribute__((address_space(3))) void *)&stage[wave][i * 64],
+                                             16, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    const float *mine = reinterpret_cast<const float *>(stage[wave]) + lane * K * 3;
+    float c[kShMaxK][3];
+    switch (deg) {
+        case 3:
+#pragma unroll
+            for (int k = 0; k < 15; ++k)
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) c[k][ch] = mine[3 * k + ch];
+            sh_forward_eval<3>(g, dcv, mv, c, V);
+            break;
+        case 2:
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) c[k][ch] = mine[3 * k + ch];
+            sh_forward_eval<2>(g, dcv, mv, c, V);
+            break;
+        default:
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) c[k][ch] = mine[3 * k + ch];
+            sh_forward_eval<1>(g, dcv, mv, c, V);
+            break;
     }
 }
 
@@ -292,6 +409,11 @@ extern "C" int gray_sh_forward(gray_ctx *ctx, const gray_params *params, const g
             hipLaunchKernelGGL(k_sh_forward<__half>, dim3(gray_div_up(n, 128)), dim3(128), 0, s, n,
                                params->num_sh_coeffs, params->current_sh_degree, knobs->enable_sh, params->mean,
                                (const __half *)params->sh_dc, (const __half *)params->sh_rest, V);
+        else if (params->num_sh_coeffs > 0 && params->num_sh_coeffs <= kShMaxK &&
+                 ((uintptr_t)params->sh_rest & 15u) == 0)
+            hipLaunchKernelGGL(k_sh_forward_lds, dim3(gray_div_up(n, 64 * kShLdsWaves)), dim3(64 * kShLdsWaves), 0, s,
+                               n, params->num_sh_coeffs, params->current_sh_degree, knobs->enable_sh, params->mean,
+                               (const float *)params->sh_dc, (const float *)params->sh_rest, V);
         else
             hipLaunchKernelGGL(k_sh_forward<float>, dim3(gray_div_up(n, 128)), dim3(128), 0, s, n,
                                params->num_sh_coeffs, params->current_sh_degree, knobs->enable_sh, params->mean,

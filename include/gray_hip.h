@@ -434,6 +434,12 @@ int gray_debug_wide(gray_ctx *ctx, uint32_t *nodes, uint8_t *flags, void *stream
  * the number of entries; counts may be NULL to query it. */
 int gray_debug_list_counts(gray_ctx *ctx, int32_t *counts, int64_t max_lanes, int64_t *lanes, void *stream);
 
+/* No reference counterpart (profiling aid).  Copies `bytes` (a multiple of
+ * 16, both pointers 16-byte aligned) from src to dst with non-temporal 16-byte
+ * loads and stores: the sustained read + write rate of a kernel that moves
+ * each byte once, the reference point quoted beside the streaming kernels. */
+int gray_debug_stream_copy(void *dst, const void *src, int64_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
