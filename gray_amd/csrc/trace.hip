@@ -1284,7 +1284,16 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_wave(TraceArgs A, const
     const float tthr = *A.knobs.t_threshold;
     const bool depth_on = *A.knobs.render_depth != 0;
     const bool h16 = A.fp16 != 0;
-    for (int q = blockIdx.x; q < total; q += gridDim.x) {
+    // XCD-aware: the queue (rays in roughly tile order) is cut into 8
+    // contiguous segments and workgroup b, which runs on XCD b % 8, takes
+    // rays of segment b % 8.  Neighbouring rays' entries share the 128-byte
+    // lines of the slot-major list, so a line one wave fetches serves the
+    // waves of its neighbours from the same L2.
+    const int seg = (total + 7) >> 3;
+    const int xcd = blockIdx.x & 7, per_xcd = (int)(gridDim.x >> 3);
+    for (int k = (int)(blockIdx.x >> 3); k < seg; k += per_xcd) {
+        const int q = xcd * seg + k;
+        if (q >= total) break;
         const uint32_t g = A.heavy_rays[q];
         const int n = A.list_n[g];
         float d[P], a[P];
