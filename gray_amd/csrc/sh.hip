@@ -361,7 +361,9 @@ __device__ __forceinline__ void sh_backward_body(int g, int K, const float *__re
     grad_mean[3 * g + 2] += gm2;
 }
 
-__global__ void k_sh_backward(int n, int K, const int32_t *__restrict__ cur_degree, const uint8_t *__restrict__ enable,
+// 128-thread launches: the bound lifts the register cap that a 1024-thread
+// default would impose (no spill of the 45 coefficient gradients)
+__global__ __launch_bounds__(128) void k_sh_backward(int n, int K, const int32_t *__restrict__ cur_degree, const uint8_t *__restrict__ enable,
                               const float *__restrict__ mean, const float *__restrict__ rest,
                               const uint8_t *__restrict__ visible, float *__restrict__ grad_dc,
                               float *__restrict__ grad_rest, float *__restrict__ grad_mean, int accumulate,

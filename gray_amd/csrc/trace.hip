@@ -1029,7 +1029,7 @@ struct RegKB {
 // entry.  After the last round one more scan sums the unflagged entries,
 // which are exactly the skip set (hits the k-buffer never handed to
 // compositing, hits beyond zfar, hits after termination).
-__global__ __launch_bounds__(GRAY_WAVE) __attribute__((amdgpu_waves_per_eu(4))) void k_composite_lists(TraceArgs A, const float *__restrict__ rgb0) {
+__global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, const float *__restrict__ rgb0) {
     int vi, tx, ty, tile;
     if (!block_tile(A, vi, tx, ty, &tile)) return;
     const DevView &V = A.views[vi];
