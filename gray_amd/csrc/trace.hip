@@ -1574,10 +1574,11 @@ __device__ __forceinline__ void backward_hit(const TraceArgs &A, const DevView &
     // scale
     const float s0 = expf(A.scale[3 * gid + 0]), s1 = expf(A.scale[3 * gid + 1]), s2 = expf(A.scale[3 * gid + 2]);
     const float *L = A.l2w + 12 * (size_t)gid;
-    const float d0 = s0 * clip + eps_sg, d1 = s1 * clip + eps_sg, d2 = s2 * clip + eps_sg;
-    const float gs0 = ((gl[0][0] * (L[0] / d0) + gl[1][0] * (L[3] / d0)) + gl[2][0] * (L[6] / d0)) * s0;
-    const float gs1 = ((gl[0][1] * (L[1] / d1) + gl[1][1] * (L[4] / d1)) + gl[2][1] * (L[7] / d1)) * s1;
-    const float gs2 = ((gl[0][2] * (L[2] / d2) + gl[1][2] * (L[5] / d2)) + gl[2][2] * (L[8] / d2)) * s2;
+    // the rotation columns L[:, j] / d_j: one reciprocal per column
+    const float i0 = 1.0f / (s0 * clip + eps_sg), i1 = 1.0f / (s1 * clip + eps_sg), i2 = 1.0f / (s2 * clip + eps_sg);
+    const float gs0 = ((gl[0][0] * (L[0] * i0) + gl[1][0] * (L[3] * i0)) + gl[2][0] * (L[6] * i0)) * s0;
+    const float gs1 = ((gl[0][1] * (L[1] * i1) + gl[1][1] * (L[4] * i1)) + gl[2][1] * (L[7] * i1)) * s1;
+    const float gs2 = ((gl[0][2] * (L[2] * i2) + gl[1][2] * (L[5] * i2)) + gl[2][2] * (L[8] * i2)) * s2;
     // rotation matrix rows
     const float g0x = gl[0][0] * s0, g0y = gl[0][1] * s1, g0z = gl[0][2] * s2;
     const float g1x = gl[1][0] * s0, g1y = gl[1][1] * s1, g1z = gl[1][2] * s2;
@@ -1585,17 +1586,19 @@ __device__ __forceinline__ void backward_hit(const TraceArgs &A, const DevView &
     const float qr = A.rotation[4 * gid + 0], qx = A.rotation[4 * gid + 1], qy = A.rotation[4 * gid + 2],
                 qz = A.rotation[4 * gid + 3];
     const float qlen = sqrtf(((qr * qr + qx * qx) + qy * qy) + qz * qz);
-    const float nr = qr / qlen, nx = qx / qlen, ny = qy / qlen, nz = qz / qlen;
+    const float iq = 1.0f / qlen;
+    const float nr = qr * iq, nx = qx * iq, ny = qy * iq, nz = qz * iq;
     const float gr = (2.f * nx * (g2y - g1z) + 2.f * ny * (g0z - g2x)) + 2.f * nz * (g1x - g0y);
     const float gx = ((-4.f * nx * (g1y + g2z) + 2.f * ny * (g0y + g1x)) + 2.f * nz * (g0z + g2x)) + 2.f * nr * (g2y - g1z);
     const float gy = ((2.f * nx * (g0y + g1x) - 4.f * ny * (g0x + g2z)) + 2.f * nz * (g1z + g2y)) + 2.f * nr * (g0z - g2x);
     const float gz = ((2.f * nx * (g0z + g2x) + 2.f * ny * (g1z + g2y)) - 4.f * nz * (g0x + g1y)) + 2.f * nr * (g1x - g0y);
+    // reference backward_normalize_act: (g - q (q.g) / |q|^2) / |q|
     const float qdot = ((gr * qr + gx * qx) + gy * qy) + gz * qz;
-    const float len3 = powf(qlen, 3.0f);
-    const float q0 = (qdot * -qr) / len3 + gr / qlen;
-    const float q1 = (qdot * -qx) / len3 + gx / qlen;
-    const float q2 = (qdot * -qy) / len3 + gy / qlen;
-    const float q3 = (qdot * -qz) / len3 + gz / qlen;
+    const float iq3 = (iq * iq) * iq;
+    const float q0 = (qdot * -qr) * iq3 + gr * iq;
+    const float q1 = (qdot * -qx) * iq3 + gx * iq;
+    const float q2 = (qdot * -qy) * iq3 + gy * iq;
+    const float q3 = (qdot * -qz) * iq3 + gz * iq;
 
     t[3] = q0;
     t[4] = q1;
