@@ -353,6 +353,16 @@ def run_train(args, traffic):
         step_ls()
         ls_ms, _ = _median_ms(step_ls, args.loss_steps)
         out["step_l1_ssim_ms_p50"] = round(ls_ms, 4)
+    # the same step with the deterministic gradient reduction (walks without
+    # continuation tasks, per-Gaussian sums in pixel order); not the metric
+    log("[bench] 10 steps with deterministic gradient sums")
+    nm.deterministic = True
+    step()
+    det_ms, _ = _median_ms(step, 10)
+    nm.deterministic = False
+    torch.cuda.synchronize()
+    nm.device_status()
+    out["step_deterministic_ms_p50"] = round(det_ms, 4)
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline on {args.cpu_rays} rays")
         out["cpu_baseline"] = cpu_baseline(scene, cam, cfg, args.cpu_rays)

@@ -676,6 +676,60 @@ __global__ void k_widen(int count, const uint8_t *__restrict__ flag, const BvhNo
     d[3] = make_uint4(ch[0], ch[1], ch[2], ch[3]);
 }
 
+}  // namespace
+
+// Block sums gray_exclusive_scan_u32 needs for m values (elements).
+int64_t gray_scan_scratch(int64_t m) { return gray_div_up(std::max<int64_t>(m, 1), kScanTile) + 1; }
+
+// Scratch sizes of gray_radix_sort_pairs for n items (elements).
+void gray_radix_scratch(int64_t n, int64_t *hist_elems, int64_t *sums_elems) {
+    const int64_t tiles = gray_div_up(std::max<int64_t>(n, 1), kSortTile);
+    *hist_elems = 256 * tiles;
+    *sums_elems = gray_scan_scratch(*hist_elems);
+}
+
+// Exclusive prefix sum of m uint32 values in place (block_sums: as many as
+// gray_radix_scratch gives for a histogram of m elements).
+int gray_exclusive_scan_u32(uint32_t *data, int m, uint32_t *block_sums, hipStream_t s) {
+    if (m <= 0) return GRAY_OK;
+    const int sblocks = (int)gray_div_up(m, kScanTile);
+    hipLaunchKernelGGL(k_scan_block, dim3(sblocks), dim3(kScanThreads), 0, s, data, m, block_sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, block_sums, sblocks);
+    hipLaunchKernelGGL(k_scan_add, dim3(sblocks), dim3(kScanThreads), 0, s, data, m, block_sums);
+    GRAY_HIP(hipGetLastError());
+    return GRAY_OK;
+}
+
+// Stable LSD radix sort of n (key, value) pairs by the low 8*passes bits of
+// the key, ping-ponging between (keys, vals) and (keys_alt, vals_alt); the
+// sorted pairs end in *keys_res / *vals_res.  Equal keys keep their input
+// order, so the result does not depend on scheduling.
+int gray_radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, int n, int passes,
+                          uint32_t *hist, uint32_t *block_sums, hipStream_t s, uint32_t **keys_res,
+                          uint32_t **vals_res) {
+    uint32_t *kin = keys, *vin = vals, *kout = keys_alt, *vout = vals_alt;
+    if (n > 0) {
+        const int tiles = (int)gray_div_up(n, kSortTile);
+        const int m = 256 * tiles;
+        for (int pass = 0; pass < passes; ++pass) {
+            const int shift = 8 * pass;
+            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, s, n, kin, shift, hist, tiles);
+            GRAY_HIP(hipGetLastError());
+            if (int rc = gray_exclusive_scan_u32(hist, m, block_sums, s)) return rc;
+            hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, n, kin, vin, kout, vout, shift,
+                               hist, tiles);
+            GRAY_HIP(hipGetLastError());
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+        }
+    }
+    *keys_res = kin;
+    *vals_res = vin;
+    return GRAY_OK;
+}
+
+namespace {
+
 template <typename T>
 int grow(T **ptr, int64_t count) {
     if (*ptr) {
@@ -783,24 +837,12 @@ static int build_topology(gray_ctx *ctx, const float *mean, hipStream_t s) {
     hipLaunchKernelGGL(k_morton, dim3(gray_div_up(n, 256)), dim3(256), 0, s, n, mean,
                        ctx->bounds_partial + 6 * 1024, ctx->keys, ctx->sorted_ids);
     GRAY_HIP(hipGetLastError());
-    // radix sort (keys, sorted_ids) in 4 passes, ping-pong with *_alt
-    const int tiles = (int)gray_div_up(n, kSortTile);
-    const int m = 256 * tiles;
-    const int sblocks = (int)gray_div_up(m, kScanTile);
-    uint32_t *kin = ctx->keys, *vin = ctx->sorted_ids, *kout = ctx->keys_alt, *vout = ctx->vals_alt;
-    for (int pass = 0; pass < 4; ++pass) {
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kSortThreads), 0, s, n, kin, shift, ctx->hist, tiles);
-        hipLaunchKernelGGL(k_scan_block, dim3(sblocks), dim3(kScanThreads), 0, s, ctx->hist, m, ctx->block_sums);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, ctx->block_sums, sblocks);
-        hipLaunchKernelGGL(k_scan_add, dim3(sblocks), dim3(kScanThreads), 0, s, ctx->hist, m, ctx->block_sums);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kSortThreads), 0, s, n, kin, vin, kout, vout, shift,
-                           ctx->hist, tiles);
-        GRAY_HIP(hipGetLastError());
-        std::swap(kin, kout);
-        std::swap(vin, vout);
-    }
-    // after an even number of passes the result is back in keys / sorted_ids
+    // radix sort (keys, sorted_ids) in 4 passes, ping-pong with *_alt; after
+    // an even number of passes the result is back in keys / sorted_ids
+    uint32_t *kres = nullptr, *vres = nullptr;
+    if (int rc = gray_radix_sort_pairs(ctx->keys, ctx->sorted_ids, ctx->keys_alt, ctx->vals_alt, (int)n, 4, ctx->hist,
+                                       ctx->block_sums, s, &kres, &vres))
+        return rc;
     GRAY_HIP(hipMemsetAsync(ctx->top_count, 0, sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_karras, dim3(gray_div_up(n - 1, 256)), dim3(256), 0, s, n, ctx->keys, ctx->sorted_ids,
                        ctx->nodes, ctx->parent_int, ctx->parent_leaf, ctx->range_lo, ctx->range_hi, ctx->top_nodes,

@@ -143,6 +143,13 @@ struct TraceArgs {
     // overflowed, so the re-traversal kernel has work (NULL: lists off)
     int32_t *retrace_any;
     int32_t wave_list_min, pad5;
+    // deterministic gradient reduction (one view per launch): the backward
+    // writes the terms of the composited hit with index k of pixel pix to
+    // record det_off[pix] + k of det_rec (16 floats: the gradient terms, then
+    // the Gaussian id bits) and the Gaussian id to det_keys
+    float *det_rec;
+    uint32_t *det_keys;
+    const uint32_t *det_off;
 };
 
 // ---------------------------------------------------------------------------

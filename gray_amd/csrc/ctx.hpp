@@ -54,6 +54,14 @@ struct gray_ctx {
     uint2 *tasks[2] = {nullptr, nullptr};
     int64_t task_cap = 0;
     int32_t *task_counters = nullptr;  // [2 * kMaxTaskRounds + 2]
+    // deterministic gradient reduction: per-hit records, their sort keys and scratch
+    float *det_rec = nullptr;        // [det_cap][16]: 15 gradient terms, Gaussian id bits
+    uint32_t *det_keys[2] = {nullptr, nullptr}, *det_vals[2] = {nullptr, nullptr};
+    int64_t det_cap = 0;
+    uint32_t *det_off = nullptr;     // [det_px_cap] per-pixel record offsets
+    int64_t det_px_cap = 0;
+    uint32_t *det_hist = nullptr, *det_sums = nullptr;
+    int64_t det_hist_cap = 0, det_sums_cap = 0;
     // timing
     bool timing = false;
     float last_build_ms = -1.0f;
@@ -68,4 +76,12 @@ struct gray_ctx {
 inline const gray_ctx *scene_of(const gray_ctx *ctx) { return ctx->scene ? ctx->scene : ctx; }
 
 int gray_ctx_reserve(gray_ctx *ctx, int64_t n);
+// sorting and scanning primitives of the BVH build, shared with the
+// deterministic gradient reduction
+int64_t gray_scan_scratch(int64_t m);
+void gray_radix_scratch(int64_t n, int64_t *hist_elems, int64_t *sums_elems);
+int gray_exclusive_scan_u32(uint32_t *data, int m, uint32_t *block_sums, hipStream_t s);
+int gray_radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt, int n, int passes,
+                          uint32_t *hist, uint32_t *block_sums, hipStream_t s, uint32_t **keys_res,
+                          uint32_t **vals_res);
 int gray_refit(gray_ctx *ctx, hipStream_t s);

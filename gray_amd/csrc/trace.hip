@@ -1470,6 +1470,17 @@ __device__ __forceinline__ void grad_to_global(const TraceArgs &A, const DevView
     if (A.visible) A.visible[gid] = 1;
 }
 
+// Deterministic mode: the terms of one composited hit go to its record (see
+// TraceArgs::det_rec); k_det_reduce sums them per Gaussian in record order.
+__device__ __forceinline__ void det_put(const TraceArgs &A, size_t idx, uint32_t gid, const float *t) {
+    f4v *o = reinterpret_cast<f4v *>(A.det_rec + 16 * idx);
+    o[0] = f4v{t[0], t[1], t[2], t[3]};
+    o[1] = f4v{t[4], t[5], t[6], t[7]};
+    o[2] = f4v{t[8], t[9], t[10], t[11]};
+    o[3] = f4v{t[12], t[13], t[14], __uint_as_float(gid)};
+    A.det_keys[idx] = gid;
+}
+
 // Per-wave gradient accumulator in LDS: the rays of a tile composite largely
 // the same Gaussians, so their terms are summed here (ds_add_f32) and each
 // Gaussian reaches the global tensors once per tile.  Open addressing with a
@@ -1721,10 +1732,40 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored(TraceArgs A
     tab.flush(A, V, lane);
 }
 
+// Deterministic mode, rays whose composited list fits the store: each hit's
+// terms go to its own record (no neighbour combining, no table, no atomics).
+__global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored_det(TraceArgs A) {
+    int vi, tx, ty;
+    if (!block_tile(A, vi, tx, ty)) return;
+    const DevView &V = A.views[vi];
+    const int lane = threadIdx.x;
+    const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
+    if (x >= V.width || y >= V.height) return;
+    const int pix = y * V.stride + x;
+    const int n = (int)V.n_comp[pix];
+    LaneRay r;
+    PixelState ps;
+    if (n <= 0 || n > A.capacity || !load_pixel_state(A, V, pix, r, ps)) return;
+    const float p = *A.knobs.exp_power, eps_sg = *A.knobs.eps_scale_grad;
+    const int R = V.ray_count;
+    const size_t base = A.det_off[pix];
+    float gT = 0.0f;
+    float Tafter = ps.Tf;
+    for (int k = n - 1; k >= 0; --k) {
+        const uint32_t g = V.hit_ids[(size_t)k * R + pix];
+        const float Ti = V.hit_T[(size_t)k * R + pix];
+        backward_hit(A, V, r, p, eps_sg, ps, g, Ti, gT,
+                     [&](uint32_t gg, const float *t) { det_put(A, base + k, gg, t); }, A.fp16 != 0, Tafter);
+    }
+}
+
 // Rays whose list exceeded the store (or capacity 0): regenerate each round by
 // re-traversal, sweep it forward (T, partial sums) and back (gradients).  The
 // gradient flowing into a round from the rounds behind it is
 // (g.C_comp - sum_{composited so far} g.c w) / T.
+// With kDet the terms of composited hit k (in distance order) go to record
+// det_off[pix] + k instead of the gradient tensors.
+template <bool kDet>
 __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArgs A, const GeoRecord *__restrict__ geo,
                                                                         const BvhNode *__restrict__ nodes) {
     __shared__ float s_kd[GRAY_KB * GRAY_WAVE];
@@ -1831,12 +1872,23 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
             const bool last = done >= n;
             if (pass == 1) {
                 float gT = last ? 0.0f : (h16 ? (Atot - P32) / T32 : (Atot - P) / T);
+                // hits 0 .. used-1 of this round were composited, after done - used earlier ones
+                const int first = done - used;
                 for (int k = used - 1; k >= 0; --k) {
                     const uint32_t gid = kb.gid(k);
                     if (gid == GRAY_EMPTY) continue;
                     float unused = 0.0f;
-                    backward_hit(A, V, r, p, eps_sg, ps, gid, kb.alpha(k), gT,
-                                 [&](uint32_t g, const float *t) { grad_to_global(A, V, g, t); }, false, unused);
+                    if (kDet) {
+                        // the forward composited n hits: records past them have no slot
+                        if (first + k >= n) continue;
+                        const size_t rec = A.det_off[pix] + (size_t)(first + k);
+                        backward_hit(A, V, r, p, eps_sg, ps, gid, kb.alpha(k), gT,
+                                     [&](uint32_t g, const float *t) { det_put(A, rec, g, t); }, false, unused);
+                    } else {
+                        backward_hit(A, V, r, p, eps_sg, ps, gid, kb.alpha(k), gT,
+                                     [&](uint32_t g, const float *t) { grad_to_global(A, V, g, t); }, false,
+                                     unused);
+                    }
                 }
             }
             alive = false;
@@ -1861,6 +1913,72 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
         scale = T32 > 0.0f ? ps.Tf / T32 : 1.0f;
     }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic gradient reduction (one view per launch)
+// ---------------------------------------------------------------------------
+// Records per pixel of view 0: its composited hits (0 outside the image);
+// entry ray_count is 0 so that the exclusive scan ends with the total.
+__global__ void k_det_counts(TraceArgs A, uint32_t *__restrict__ cnt) {
+    const DevView &V = A.views[0];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > V.ray_count) return;
+    uint32_t c = 0;
+    if (i < V.ray_count) {
+        const int x = i % V.stride, y = i / V.stride;
+        if (x < V.width && y < V.height) c = V.n_comp[i];
+    }
+    cnt[i] = c;
+}
+
+// Records not written by the backward (rays it skips) keep key `none`, which
+// sorts after every Gaussian id; values are the record indices.
+__global__ void k_det_init(uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, int total, uint32_t none) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    keys[i] = none;
+    vals[i] = (uint32_t)i;
+}
+
+// One thread per run of equal keys in the sorted records: sums the run's
+// terms in record order (pixel, then hit order) and adds them to the
+// gradient tensors.  Runs are disjoint, so no atomics are needed.
+__global__ void k_det_reduce(TraceArgs A, const uint32_t *__restrict__ keys, const uint32_t *__restrict__ vals,
+                             int total) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const uint32_t gid = keys[i];
+    if (gid >= (uint32_t)A.num_gaussians) return;
+    if (i > 0 && keys[i - 1] == gid) return;
+    float t[kGradTerms];
+#pragma unroll
+    for (int j = 0; j < kGradTerms; ++j) t[j] = 0.0f;
+    for (int j = i; j < total && keys[j] == gid; ++j) {
+        const f4v *rec = reinterpret_cast<const f4v *>(A.det_rec + 16 * (size_t)vals[j]);
+        const f4v a = rec[0], b = rec[1], c = rec[2], d = rec[3];
+        t[0] += a.x; t[1] += a.y; t[2] += a.z; t[3] += a.w;
+        t[4] += b.x; t[5] += b.y; t[6] += b.z; t[7] += b.w;
+        t[8] += c.x; t[9] += c.y; t[10] += c.z; t[11] += c.w;
+        t[12] += d.x; t[13] += d.y; t[14] += d.z;
+    }
+    const DevView &V = A.views[0];
+    V.grad_rgb[3 * gid + 0] += t[0];
+    V.grad_rgb[3 * gid + 1] += t[1];
+    V.grad_rgb[3 * gid + 2] += t[2];
+    A.g_rot[4 * gid + 0] += t[3];
+    A.g_rot[4 * gid + 1] += t[4];
+    A.g_rot[4 * gid + 2] += t[5];
+    A.g_rot[4 * gid + 3] += t[6];
+    A.g_scale[3 * gid + 0] += t[7];
+    A.g_scale[3 * gid + 1] += t[8];
+    A.g_scale[3 * gid + 2] += t[9];
+    A.g_mean[3 * gid + 0] += t[10];
+    A.g_mean[3 * gid + 1] += t[11];
+    A.g_mean[3 * gid + 2] += t[12];
+    A.g_op[gid] += t[13];
+    if (A.g_prune) A.g_prune[gid] += t[14];
+    if (A.visible) A.visible[gid] = 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -2120,6 +2238,84 @@ int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
     return GRAY_OK;
 }
 
+// Grows a device buffer to `count` elements (contents are not kept).
+template <typename T>
+int det_grow(T *&buf, int64_t &cap, int64_t count, const char *what) {
+    if (count <= cap) return GRAY_OK;
+    GRAY_HIP(hipFree(buf));
+    buf = nullptr;
+    cap = 0;
+    if (hipMalloc(&buf, sizeof(T) * count) != hipSuccess) {
+        (void)hipGetLastError();
+        gray_set_error(GRAY_ERR_NOMEM, "render_backward(deterministic): cannot allocate %s (%lld bytes)", what,
+                       (long long)(sizeof(T) * count));
+        return GRAY_ERR_NOMEM;
+    }
+    cap = count;
+    return GRAY_OK;
+}
+
+// Deterministic backward of one view (A holds that view alone): per-pixel
+// record offsets by a scan of the composited counts, the records of every
+// composited hit, a stable radix sort of (Gaussian id, record index) and one
+// ordered sum per Gaussian.  Reads the record total back to the host.
+int det_backward(gray_ctx *ctx, TraceArgs &A, hipStream_t s) {
+    const DevView &V = A.views[0];
+    const int64_t m = (int64_t)V.ray_count + 1;
+    int rc;
+    if ((rc = det_grow(ctx->det_off, ctx->det_px_cap, m, "record offsets"))) return rc;
+    if ((rc = det_grow(ctx->det_sums, ctx->det_sums_cap, gray_scan_scratch(m), "scan sums"))) return rc;
+    hipLaunchKernelGGL(k_det_counts, dim3(gray_div_up(m, 256)), dim3(256), 0, s, A, ctx->det_off);
+    GRAY_HIP(hipGetLastError());
+    if ((rc = gray_exclusive_scan_u32(ctx->det_off, (int)m, ctx->det_sums, s))) return rc;
+    uint32_t total = 0;
+    GRAY_HIP(hipMemcpyAsync(&total, ctx->det_off + (m - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    GRAY_HIP(hipStreamSynchronize(s));
+    if (total > (uint32_t)INT32_MAX) {
+        gray_set_error(GRAY_ERR_INVALID, "render_backward(deterministic): %u composited hits exceed 2^31 - 1", total);
+        return GRAY_ERR_INVALID;
+    }
+    if (total == 0) return GRAY_OK;
+    const int64_t recs = total;
+    if (recs > ctx->det_cap) {
+        int64_t c[5] = {0, 0, 0, 0, 0};
+        ctx->det_cap = 0;
+        if ((rc = det_grow(ctx->det_rec, c[0], 16 * recs, "gradient records"))) return rc;
+        for (int i = 0; i < 2; ++i) {
+            if ((rc = det_grow(ctx->det_keys[i], c[1 + i], recs, "record keys"))) return rc;
+            if ((rc = det_grow(ctx->det_vals[i], c[3 + i], recs, "record indices"))) return rc;
+        }
+        ctx->det_cap = recs;
+    }
+    int64_t hist = 0, sums = 0;
+    gray_radix_scratch(recs, &hist, &sums);
+    if ((rc = det_grow(ctx->det_hist, ctx->det_hist_cap, hist, "sort histogram"))) return rc;
+    if ((rc = det_grow(ctx->det_sums, ctx->det_sums_cap, sums, "scan sums"))) return rc;
+    const uint32_t none = (uint32_t)A.num_gaussians;  // sorts after every id
+    hipLaunchKernelGGL(k_det_init, dim3(gray_div_up(recs, 256)), dim3(256), 0, s, ctx->det_keys[0], ctx->det_vals[0],
+                       (int)recs, none);
+    GRAY_HIP(hipGetLastError());
+    A.det_rec = ctx->det_rec;
+    A.det_keys = ctx->det_keys[0];
+    A.det_off = ctx->det_off;
+    const dim3 grid(8 * A.xcd_chunk), block(GRAY_WAVE);
+    if (A.capacity > 0) {
+        hipLaunchKernelGGL(k_trace_backward_stored_det, grid, block, 0, s, A);
+        GRAY_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_trace_backward_recompute<true>, grid, block, 0, s, A, A.geo, A.nodes);
+    GRAY_HIP(hipGetLastError());
+    int bits = 0;
+    while (bits < 32 && (none >> bits) != 0) ++bits;
+    uint32_t *keys = nullptr, *vals = nullptr;
+    if ((rc = gray_radix_sort_pairs(ctx->det_keys[0], ctx->det_vals[0], ctx->det_keys[1], ctx->det_vals[1],
+                                    (int)recs, (bits + 7) / 8, ctx->det_hist, ctx->det_sums, s, &keys, &vals)))
+        return rc;
+    hipLaunchKernelGGL(k_det_reduce, dim3(gray_div_up(recs, 256)), dim3(256), 0, s, A, keys, vals, (int)recs);
+    GRAY_HIP(hipGetLastError());
+    return GRAY_OK;
+}
+
 int check_render(gray_ctx *ctx, const gray_render_args *args) {
     if (!ctx || !args || !args->views || args->num_views <= 0) {
         gray_set_error(GRAY_ERR_INVALID, "render: null context/arguments or no views");
@@ -2180,13 +2376,16 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             GRAY_HIP(hipGetLastError());
             A.task_out = ctx->tasks[0];
             A.task_count_out = count;
-            A.budget = args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
+            // deterministic: every walk runs to its end in its own lane (a stack
+            // overflow sends the ray to re-traversal), so each list has one order
+            const bool tasks = !args->deterministic;
+            A.budget = !tasks ? -1 : args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
             if ((rc = mark(0))) return rc;
             const Bvh4Node *cnodes = scene_of(ctx)->wide;
             hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, cnodes);
             GRAY_HIP(hipGetLastError());
             if ((rc = mark(1))) return rc;
-            for (int round = 0; round < kTaskRounds; ++round) {
+            for (int round = 0; tasks && round < kTaskRounds; ++round) {
                 A.task_in = ctx->tasks[round & 1];
                 A.task_count_in = count + round;
                 A.task_head = head + round;
@@ -2285,8 +2484,10 @@ extern "C" int gray_render_backward(gray_ctx *ctx, const gray_render_args *args,
     }
     if (n_scene == 0) return GRAY_OK;
     hipStream_t s = (hipStream_t)stream;
-    for (int first = 0; first < args->num_views; first += GRAY_MAX_VIEWS) {
-        const int nv = std::min(GRAY_MAX_VIEWS, args->num_views - first);
+    // deterministic: one view per launch, so each Gaussian's sum runs in one thread
+    const int chunk = args->deterministic ? 1 : GRAY_MAX_VIEWS;
+    for (int first = 0; first < args->num_views; first += chunk) {
+        const int nv = std::min(chunk, args->num_views - first);
         TraceArgs A;
         if ((rc = fill_args(ctx, args, first, nv, A))) return rc;
         for (int i = 0; i < nv; ++i) {
@@ -2308,11 +2509,15 @@ extern "C" int gray_render_backward(gray_ctx *ctx, const gray_render_args *args,
         A.g_prune = grads->pruning_weight;
         A.visible = grads->was_visible;
         if (A.total_tiles == 0) continue;
+        if (args->deterministic) {
+            if ((rc = det_backward(ctx, A, s))) return rc;
+            continue;
+        }
         if (A.capacity > 0) {
             hipLaunchKernelGGL(k_trace_backward_stored, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A);
             GRAY_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(k_trace_backward_recompute, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, A.geo,
+        hipLaunchKernelGGL(k_trace_backward_recompute<false>, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, A.geo,
                            A.nodes);
         GRAY_HIP(hipGetLastError());
     }
@@ -2328,8 +2533,10 @@ extern "C" int gray_render_ellipsoids(gray_ctx *ctx, const gray_render_args *arg
         return GRAY_ERR_INVALID;
     }
     hipStream_t s = (hipStream_t)stream;
-    for (int first = 0; first < args->num_views; first += GRAY_MAX_VIEWS) {
-        const int nv = std::min(GRAY_MAX_VIEWS, args->num_views - first);
+    // deterministic: one view per launch, so each Gaussian's sum runs in one thread
+    const int chunk = args->deterministic ? 1 : GRAY_MAX_VIEWS;
+    for (int first = 0; first < args->num_views; first += chunk) {
+        const int nv = std::min(chunk, args->num_views - first);
         TraceArgs A;
         if ((rc = fill_args(ctx, args, first, nv, A))) return rc;
         for (int i = 0; i < nv; ++i) {

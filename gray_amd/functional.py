@@ -49,6 +49,7 @@ class RenderSettings:
     list_capacity: Optional[int] = None   # None: from the memory cap
     fp16_hits: bool = False               # the reference's fp16 hit storage (distances, compositing alphas)
     hit_capacity: Optional[int] = None    # None: from the memory cap
+    deterministic: bool = False           # bit-reproducible gradients (ordered sums instead of atomics; slower)
 
 
 def capacities(views: int, width: int, height: int, cap_bytes: int):
@@ -162,6 +163,7 @@ class MultiViewRenderer:
         a.save = 1 if save else 0
         a.hit_capacity, a.list_capacity = self.last_capacities
         a.fp16_hits = 1 if self.settings.fp16_hits else 0
+        a.deterministic = 1 if self.settings.deterministic else 0
         a.use_bvh = 1
         return a
 

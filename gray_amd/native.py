@@ -333,6 +333,9 @@ class NativeRaytracer:
         # (per_pixel_linked_list.h:28-29); False: fp32 throughout
         self.fp16_hits = False
         self.wave_list_min = 0  # lists longer than this are composited by a whole wave (0: built-in, < 0: never)
+        # reproducible gradients: walks without continuation tasks and
+        # per-Gaussian sums in pixel order instead of atomics (slower)
+        self.deterministic = False
         self.tile_stats = None  # optional int32 [tiles, 4] traversal statistics of the forward
         self.timer = _NoTimer()
         self._built_count = -1
@@ -395,6 +398,7 @@ class NativeRaytracer:
         a.collect_budget = int(self.collect_budget)
         a.fp16_hits = 1 if self.fp16_hits else 0
         a.wave_list_min = int(self.wave_list_min)
+        a.deterministic = 1 if self.deterministic else 0
         return a
 
     def _require_bvh(self) -> None:

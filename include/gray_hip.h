@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 11
+#define GRAY_ABI_VERSION 12
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -141,6 +141,14 @@ typedef struct {
                                    a whole wave each instead of one lane: same composited hits, transmittance
                                    and colour; the skip-set sums are added in another order.  0 = the
                                    built-in default (96), < 0 = never */
+    int32_t deterministic;      /* 1 = reproducible gradients: the forward walks each ray's tree in one lane
+                                   (no continuation tasks), so hit lists and truncation sums keep one order,
+                                   and the backward sums each Gaussian's terms in pixel and hit order
+                                   (records sorted by Gaussian with a stable radix sort) instead of with
+                                   atomics.  Two runs on the same inputs give bit-identical images and
+                                   gradients.  The backward then synchronizes the stream once per view and
+                                   keeps 80 bytes per composited hit.  No counterpart in the reference
+                                   (its gradients are summed with atomics); 0 = off */
 } gray_render_args;
 
 /* Gradient targets of the backward (reference gaussians.h:34-62). */
