@@ -375,15 +375,11 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 // Collect pass: every hit of every ray into its list
 // ---------------------------------------------------------------------------
 // Tuning of the collect pass (compile-time; -D overrides for sweeps).
-// Stack entries per lane (LDS).  When it is full, a walk that may hand work
-// on spills the entry as a task; the last task round, which may not, has the
-// larger stack and falls back to re-traversal on overflow, so the bounds cost
-// speed, never results.
+// Stack entries per lane (LDS).  When it is full, a walk spills the entry as
+// a task; the last task round (pooled, below) falls back to re-traversal when
+// its stack overflows, so the bounds cost speed, never results.
 #ifndef GRAY_COLLECT_STACK
 #define GRAY_COLLECT_STACK 16
-#endif
-#ifndef GRAY_FINAL_STACK
-#define GRAY_FINAL_STACK 64
 #endif
 // Leaves tested per lane and iteration.
 #ifndef GRAY_COLLECT_LEAVES
@@ -400,16 +396,24 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 #define GRAY_COLLECT_BUDGET 192
 #endif
 #ifndef GRAY_TASK_BUDGET
-#define GRAY_TASK_BUDGET 48
+#define GRAY_TASK_BUDGET 64
+#endif
+// Tasks a wave of the pooled last round (k_collect_pool) takes at a time, and
+// its LDS stack.
+#ifndef GRAY_POOL_TAKE
+#define GRAY_POOL_TAKE 32
+#endif
+#ifndef GRAY_POOL_STACK
+#define GRAY_POOL_STACK 1024
 #endif
 #ifndef GRAY_TASK_ROUNDS
-#define GRAY_TASK_ROUNDS 4
+#define GRAY_TASK_ROUNDS 3
 #endif
 constexpr int kCollectStack = GRAY_COLLECT_STACK;
-constexpr int kFinalStack = GRAY_FINAL_STACK;
 constexpr int kCollectLeaves = GRAY_COLLECT_LEAVES;
 constexpr int kLeafQueue = GRAY_LEAF_QUEUE;
-constexpr int kTaskRounds = GRAY_TASK_ROUNDS;
+constexpr int kTaskRounds = GRAY_TASK_ROUNDS;  // the last one pooled
+static_assert(kTaskRounds >= 1, "at least the pooled task round");
 constexpr int kTaskWaves = 4096;           // persistent waves of a task round
 constexpr int32_t kRayDead = -(1 << 30);   // list_n of a ray whose list or tasks overflowed
 static_assert((kLeafQueue & (kLeafQueue - 1)) == 0, "leaf queue size: a power of two");
@@ -703,6 +707,135 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const 
         collect_walk<true, kStack>(A, geo, nodes, r, g, start, A.budget, s_stack, s_leaf, lane, cnt, ok, visits,
                                   leaves);
         if (!ok && start != GRAY_EMPTY) atomicExch(A.list_n + g, kRayDead);
+    }
+}
+
+// Pooled task round: a wave takes kPoolTake tasks and its 64 lanes share
+// their work.  The wave's frontier (node or leaf references, each tagged with
+// its task's slot) is one LDS stack; every step pops the top 64 entries, one
+// per lane, whatever task they belong to: a leaf gets the exact test (hits
+// appended to the ray's list with an atomic slot), a node its four child box
+// tests, and the hit children are pushed.  So a task with a large subtree is
+// walked 64 entries at a time instead of by one lane, and the round's length
+// follows the tree depth and the wave's total work rather than its longest
+// walk.  An entry that does not fit the stack marks its ray for re-traversal.
+constexpr int kPoolTake = GRAY_POOL_TAKE;
+constexpr int kPoolStack = GRAY_POOL_STACK;
+static_assert(kPoolTake >= 1 && kPoolTake <= GRAY_WAVE, "pool take: 1 to 64 tasks");
+static_assert(kPoolStack >= GRAY_WAVE, "pool stack: at least one entry per lane");
+
+__global__ __launch_bounds__(GRAY_WAVE) void k_collect_pool(TraceArgs A, const GeoRecord *__restrict__ geo,
+                                                            const Bvh4Node *__restrict__ wide) {
+    __shared__ uint32_t s_ref[kPoolStack];
+    __shared__ uint8_t s_slot[kPoolStack];
+    __shared__ float s_ray[9 * kPoolTake];  // [component][slot]: origin, direction, inverse direction
+    __shared__ uint32_t s_g[kPoolTake];
+    const int lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int total = min(*A.task_count_in, A.task_cap);
+    const int cap = A.list_cap;
+    const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power;
+    float one = 1.0f;
+    asm volatile("" : "+v"(one));
+    for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(A.task_head, kPoolTake);
+        base = __shfl(base, 0);
+        if (base >= total) break;
+        uint32_t start = GRAY_EMPTY;
+        if (lane < kPoolTake) {
+            const int t = base + lane;
+            uint32_t g = 0;
+            LaneRay r{};
+            if (t < total) {
+                const uint2 task = A.task_in[t];
+                if (task.x < (uint64_t)A.list_stride) {
+                    g = task.x;
+                    int x, y;
+                    const DevView &V = lane_pixel(A, g, x, y);
+                    if (x < V.width && y < V.height && A.list_n[g] >= 0 && make_ray(A.knobs, V, x, y, r))
+                        start = task.y;
+                }
+            }
+            s_ray[0 * kPoolTake + lane] = r.ox;
+            s_ray[1 * kPoolTake + lane] = r.oy;
+            s_ray[2 * kPoolTake + lane] = r.oz;
+            s_ray[3 * kPoolTake + lane] = r.dx;
+            s_ray[4 * kPoolTake + lane] = r.dy;
+            s_ray[5 * kPoolTake + lane] = r.dz;
+            s_ray[6 * kPoolTake + lane] = r.ix;
+            s_ray[7 * kPoolTake + lane] = r.iy;
+            s_ray[8 * kPoolTake + lane] = r.iz;
+            s_g[lane] = g;
+        }
+        const uint64_t has = __ballot(start != GRAY_EMPTY);
+        if (start != GRAY_EMPTY) {
+            const int o = __popcll(has & below);
+            s_ref[o] = start;
+            s_slot[o] = (uint8_t)lane;
+        }
+        int sp = __popcll(has);
+        __syncthreads();
+        while (sp > 0) {
+            // pop the top entries, one per lane
+            const int n = min(sp, GRAY_WAVE);
+            sp -= n;
+            uint32_t ref = GRAY_EMPTY;
+            int slot = 0;
+            if (lane < n) {
+                ref = s_ref[sp + lane];
+                slot = s_slot[sp + lane];
+            }
+            uint32_t ch[4] = {GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY};
+            uint32_t g = 0;
+            if (ref != GRAY_EMPTY) {
+                LaneRay q;
+                q.ox = s_ray[0 * kPoolTake + slot];
+                q.oy = s_ray[1 * kPoolTake + slot];
+                q.oz = s_ray[2 * kPoolTake + slot];
+                q.dx = s_ray[3 * kPoolTake + slot];
+                q.dy = s_ray[4 * kPoolTake + slot];
+                q.dz = s_ray[5 * kPoolTake + slot];
+                q.ix = s_ray[6 * kPoolTake + slot];
+                q.iy = s_ray[7 * kPoolTake + slot];
+                q.iz = s_ray[8 * kPoolTake + slot];
+                q.one = one;
+                g = s_g[slot];
+                if (ref & GRAY_LEAF) {
+                    const uint32_t gid = ref & ~GRAY_LEAF;
+                    const GeoV gv = load_geo(geo, gid);
+                    int cnt = 0;
+                    bool ok = true;
+                    collect_leaf<true>(A, gv, gid, q, g, cap, athr, p, cnt, ok);
+                    if (!ok) atomicExch(A.list_n + g, kRayDead);
+                } else {
+                    const Node4V nd = load_node4(wide, ref);
+                    ch[0] = child_hit<0>(q, nd) ? nd.ch.x : GRAY_EMPTY;
+                    ch[1] = child_hit<1>(q, nd) ? nd.ch.y : GRAY_EMPTY;
+                    ch[2] = child_hit<2>(q, nd) ? nd.ch.z : GRAY_EMPTY;
+                    ch[3] = child_hit<3>(q, nd) ? nd.ch.w : GRAY_EMPTY;
+                }
+            }
+            // every lane has read its entry before any push overwrites it
+            __syncthreads();
+            bool lost = false;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t m = __ballot(ch[k] != GRAY_EMPTY);
+                if (ch[k] != GRAY_EMPTY) {
+                    const int o = sp + __popcll(m & below);
+                    if (o < kPoolStack) {
+                        s_ref[o] = ch[k];
+                        s_slot[o] = (uint8_t)slot;
+                    } else {
+                        lost = true;
+                    }
+                }
+                sp = min(sp + __popcll(m), kPoolStack);
+            }
+            if (lost) atomicExch(A.list_n + g, kRayDead);
+            __syncthreads();
+        }
     }
 }
 
@@ -2385,20 +2518,19 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, cnodes);
             GRAY_HIP(hipGetLastError());
             if ((rc = mark(1))) return rc;
+            // per-lane budgeted rounds, then the pooled round that finishes
             for (int round = 0; tasks && round < kTaskRounds; ++round) {
                 A.task_in = ctx->tasks[round & 1];
                 A.task_count_in = count + round;
                 A.task_head = head + round;
                 A.task_out = ctx->tasks[(round + 1) & 1];
                 A.task_count_out = count + round + 1;
-                A.budget = round + 1 < kTaskRounds ? (args->collect_budget > 0 ? args->collect_budget : GRAY_TASK_BUDGET)
-                                                   : -1;
-                if (A.budget >= 0)
+                A.budget = args->collect_budget > 0 ? args->collect_budget : GRAY_TASK_BUDGET;
+                if (round + 1 < kTaskRounds)
                     hipLaunchKernelGGL(k_collect_tasks<kCollectStack>, dim3(kTaskWaves), block, 0, s, A, A.geo,
                                        cnodes);
                 else
-                    hipLaunchKernelGGL(k_collect_tasks<kFinalStack>, dim3(kTaskWaves), block, 0, s, A, A.geo,
-                                       cnodes);
+                    hipLaunchKernelGGL(k_collect_pool, dim3(kTaskWaves), block, 0, s, A, A.geo, cnodes);
                 GRAY_HIP(hipGetLastError());
             }
             if ((rc = mark(2))) return rc;
