@@ -430,9 +430,26 @@ __device__ __forceinline__ float fp16_round(float x, bool on) { return on ? __ha
 // Hit records are written once and read back only by the compositing pass:
 // a non-temporal store (a wave writes 1 KB of whole lines per slot) keeps them
 // from evicting the nodes and leaf records that the walk re-reads.
+// Hit-list layout.  Entries below kSlotMajor are slot-major: entry k of lane
+// g at k * S + g (S = lanes of the launch), so a wave reading one entry of each
+// of its 64 rays reads 1 KB of whole lines.  Entries from kSlotMajor on, which
+// only long lists have, follow ray-major: entry k of lane g at
+// kSlotMajor * S + g * (cap - kSlotMajor) + (k - kSlotMajor), so the wave that
+// composites one long list reads whole lines of it.
+#ifndef GRAY_SLOT_MAJOR
+#define GRAY_SLOT_MAJOR 64
+#endif
+constexpr int kSlotMajor = GRAY_SLOT_MAJOR;
+
+__device__ __forceinline__ size_t list_at(const TraceArgs &A, size_t g, int k) {
+    const size_t S = (size_t)A.list_stride;
+    if (k < kSlotMajor) return (size_t)k * S + g;
+    return (size_t)kSlotMajor * S + g * (size_t)(A.list_cap - kSlotMajor) + (size_t)(k - kSlotMajor);
+}
+
 typedef float f4v __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void put_hit(const TraceArgs &A, size_t g, int slot, float dist, uint32_t gid, float a) {
-    const size_t e = (size_t)slot * (size_t)A.list_stride + g;
+    const size_t e = list_at(A, g, slot);
     const f4v rec = {dist, __uint_as_float(gid), a, 0.0f};
     __builtin_nontemporal_store(rec, reinterpret_cast<f4v *>(A.list_rec + e));
 }
@@ -1191,7 +1208,6 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
     active = active && !heavy;
     if (!__ballot(active)) return;
 
-    const size_t S = (size_t)A.list_stride;
     float4 *__restrict__ L = A.list_rec;
     const float tthr = *A.knobs.t_threshold;
     const float znear = *V.znear, zfar = *V.zfar;
@@ -1225,7 +1241,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
         auto fetch = [&](int i0) {  // unconditional loads (clamped index): exact wait counts
 #pragma unroll
             for (int j = 0; j < kListBatch; ++j) {
-                const float4 rec = L[(size_t)min(i0 + j, n - 1) * S + lg];
+                const float4 rec = L[list_at(A, lg, min(i0 + j, n - 1))];
                 nx[j] = i0 + j < n ? make_float2(rec.x, rec.y) : make_float2(INFINITY, __uint_as_float(0xFFFFFFFFu));
             }
         };
@@ -1256,7 +1272,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
             tmin = fmaxf(dk, tmin);
             if (!(dk < zfar)) continue;
             const uint32_t gid = kb.id[k];
-            const size_t e = (size_t)kb.slot[k] * S + lg;
+            const size_t e = list_at(A, lg, (int)kb.slot[k]);
             const float a32 = L[e].z;
             const float a = fp16_round(a32, h16);
             const float c0 = rgb[3 * gid + 0], c1 = rgb[3 * gid + 1], c2 = rgb[3 * gid + 2];
@@ -1309,7 +1325,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
     auto fetch_skip = [&](int i0) {
 #pragma unroll
         for (int j = 0; j < kListBatch; ++j) {
-            const float4 rec = L[(size_t)min(i0 + j, nskip - 1) * S + lg];
+            const float4 rec = L[list_at(A, lg, min(i0 + j, nskip - 1))];
             sx[j] = make_float2(rec.y, i0 + j < nskip ? rec.z : 0.0f);
         }
     };
@@ -1412,7 +1428,6 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_wave(TraceArgs A, const
     __shared__ float4 stage[P][GRAY_WAVE];  // {alpha (fp32), colour} of entry j of each lane
     const int lane = threadIdx.x;
     const int total = *A.heavy_count;
-    const size_t S = (size_t)A.list_stride;
     const float4 *__restrict__ L = A.list_rec;
     const float tthr = *A.knobs.t_threshold;
     const bool depth_on = *A.knobs.render_depth != 0;
@@ -1439,7 +1454,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_wave(TraceArgs A, const
             a[j] = 0.0f;
             js[j] = (uint32_t)j;
             if (i < n) {
-                const float4 rec = L[(size_t)i * S + g];
+                const float4 rec = L[list_at(A, g, i)];
                 d[j] = rec.x;
                 id[j] = __float_as_uint(rec.y) & ~kListMark;
                 a[j] = rec.z;
