@@ -190,7 +190,7 @@ def _counter_means(csv_path: str, counter: str):
             if row["Counter_Name"] != counter:
                 continue
             name = row["Kernel_Name"]
-            if f"{DOMINANT_KERNEL}(" in name:
+            if f"{DOMINANT_KERNEL}(" in name or f"{DOMINANT_KERNEL}<" in name:
                 kern.append(float(row["Counter_Value"]))
             elif "neg_kernel" in name:
                 cal.append(float(row["Counter_Value"]))
@@ -353,16 +353,18 @@ def run_train(args, traffic):
         step_ls()
         ls_ms, _ = _median_ms(step_ls, args.loss_steps)
         out["step_l1_ssim_ms_p50"] = round(ls_ms, 4)
-    # the same step with the deterministic gradient reduction (walks without
-    # continuation tasks, per-Gaussian sums in pixel order); not the metric
-    log("[bench] 10 steps with deterministic gradient sums")
-    nm.deterministic = True
-    step()
-    det_ms, _ = _median_ms(step, 10)
-    nm.deterministic = False
-    torch.cuda.synchronize()
-    nm.device_status()
-    out["step_deterministic_ms_p50"] = round(det_ms, 4)
+    if not args.no_extra:
+        # the same step with the deterministic gradient reduction (walks
+        # without continuation tasks, per-Gaussian sums in pixel order); not
+        # the metric
+        log("[bench] 10 steps with deterministic gradient sums")
+        nm.deterministic = True
+        step()
+        det_ms, _ = _median_ms(step, 10)
+        nm.deterministic = False
+        torch.cuda.synchronize()
+        nm.device_status()
+        out["step_deterministic_ms_p50"] = round(det_ms, 4)
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline on {args.cpu_rays} rays")
         out["cpu_baseline"] = cpu_baseline(scene, cam, cfg, args.cpu_rays)

@@ -640,10 +640,14 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
 
 static_assert(kLeafQueue >= 4, "a 4-wide visit may queue four leaves");
 
-// Per-tile collect pass: every ray of the tile walks from the root.
+// Per-tile collect pass: every ray of the tile walks from the root.  kStack
+// is kCollectStack, or kDetStack in the deterministic mode, whose walks run to
+// their end without handing work on.
+constexpr int kDetStack = 64;
+template <int kStack>
 __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRecord *__restrict__ geo,
                                                        const Bvh4Node *__restrict__ nodes) {
-    __shared__ uint32_t s_stack[kCollectStack * GRAY_WAVE];
+    __shared__ uint32_t s_stack[kStack * GRAY_WAVE];
     __shared__ uint32_t s_leaf[kLeafQueue * GRAY_WAVE];
     int vi, tx, ty, tile;
     if (!block_tile(A, vi, tx, ty, &tile)) return;
@@ -656,7 +660,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
     const size_t g = (size_t)tile * GRAY_WAVE + lane;
     int cnt = 0, visits = 0, leaves = 0;
     bool ok = true;
-    collect_walk<false, kCollectStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
+    collect_walk<false, kStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
                                       A.budget, s_stack, s_leaf, lane, cnt, ok, visits, leaves);
     A.list_n[g] = ok ? cnt : -1;
     if (A.tile_stats) {
@@ -2524,13 +2528,17 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             GRAY_HIP(hipGetLastError());
             A.task_out = ctx->tasks[0];
             A.task_count_out = count;
-            // deterministic: every walk runs to its end in its own lane (a stack
-            // overflow sends the ray to re-traversal), so each list has one order
+            // deterministic: every walk runs to its end in its own lane with a
+            // 64-entry stack (an overflow sends the ray to re-traversal), so each
+            // list has one order
             const bool tasks = !args->deterministic;
             A.budget = !tasks ? -1 : args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
             if ((rc = mark(0))) return rc;
             const Bvh4Node *cnodes = scene_of(ctx)->wide;
-            hipLaunchKernelGGL(k_collect, grid, block, 0, s, A, A.geo, cnodes);
+            if (tasks)
+                hipLaunchKernelGGL(k_collect<kCollectStack>, grid, block, 0, s, A, A.geo, cnodes);
+            else
+                hipLaunchKernelGGL(k_collect<kDetStack>, grid, block, 0, s, A, A.geo, cnodes);
             GRAY_HIP(hipGetLastError());
             if ((rc = mark(1))) return rc;
             // per-lane budgeted rounds, then the pooled round that finishes
