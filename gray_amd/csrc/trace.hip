@@ -390,31 +390,45 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 #ifndef GRAY_LEAF_QUEUE
 #define GRAY_LEAF_QUEUE 8
 #endif
-// Iterations a walk runs before it hands its remaining work on as tasks, in
-// the per-tile pass and in every task round but the last (which finishes).
+// Iterations a walk runs before it hands its remaining work on as tasks: the
+// per-tile pass (GRAY_COLLECT_BUDGET) and the optional per-lane task rounds
+// (GRAY_TASK_BUDGET).
 #ifndef GRAY_COLLECT_BUDGET
-#define GRAY_COLLECT_BUDGET 192
+#define GRAY_COLLECT_BUDGET 128
 #endif
 #ifndef GRAY_TASK_BUDGET
 #define GRAY_TASK_BUDGET 64
 #endif
-// Tasks a wave of the pooled last round (k_collect_pool) takes at a time, and
-// its LDS stack.
+// Task rounds after the first walk: per-lane budgeted rounds
+// (k_collect_tasks; none by default), then pooled rounds (k_collect_pool).
+// A pooled round finishes every task it takes; entries that overflow its
+// stacks are handed to the next pooled round, and the last one sends their
+// rays to re-traversal instead.
+#ifndef GRAY_LANE_ROUNDS
+#define GRAY_LANE_ROUNDS 0
+#endif
+#ifndef GRAY_POOL_ROUNDS
+#define GRAY_POOL_ROUNDS 2
+#endif
+// Tasks a wave of a pooled round takes at a time, and its LDS stacks.
 #ifndef GRAY_POOL_TAKE
 #define GRAY_POOL_TAKE 32
 #endif
 #ifndef GRAY_POOL_STACK
 #define GRAY_POOL_STACK 1024
 #endif
-#ifndef GRAY_TASK_ROUNDS
-#define GRAY_TASK_ROUNDS 3
-#endif
+
 constexpr int kCollectStack = GRAY_COLLECT_STACK;
 constexpr int kCollectLeaves = GRAY_COLLECT_LEAVES;
 constexpr int kLeafQueue = GRAY_LEAF_QUEUE;
-constexpr int kTaskRounds = GRAY_TASK_ROUNDS;  // the last one pooled
-static_assert(kTaskRounds >= 1, "at least the pooled task round");
-constexpr int kTaskWaves = 4096;           // persistent waves of a task round
+constexpr int kLaneRounds = GRAY_LANE_ROUNDS;
+constexpr int kTaskRounds = GRAY_LANE_ROUNDS + GRAY_POOL_ROUNDS;
+static_assert(GRAY_LANE_ROUNDS >= 0 && GRAY_POOL_ROUNDS >= 1, "at least one pooled task round");
+#ifndef GRAY_TASK_WAVES
+#define GRAY_TASK_WAVES 4096
+#endif
+constexpr int kTaskWaves = GRAY_TASK_WAVES;  // persistent waves of a task round
+constexpr int kPoolSpillWaves = 256;          // persistent waves of a pooled round after the first
 constexpr int32_t kRayDead = -(1 << 30);   // list_n of a ray whose list or tasks overflowed
 static_assert((kLeafQueue & (kLeafQueue - 1)) == 0, "leaf queue size: a power of two");
 
@@ -732,23 +746,42 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const 
 }
 
 // Pooled task round: a wave takes kPoolTake tasks and its 64 lanes share
-// their work.  The wave's frontier (node or leaf references, each tagged with
-// its task's slot) is one LDS stack; every step pops the top 64 entries, one
-// per lane, whatever task they belong to: a leaf gets the exact test (hits
-// appended to the ray's list with an atomic slot), a node its four child box
-// tests, and the hit children are pushed.  So a task with a large subtree is
-// walked 64 entries at a time instead of by one lane, and the round's length
-// follows the tree depth and the wave's total work rather than its longest
-// walk.  An entry that does not fit the stack marks its ray for re-traversal.
+// their work.  The wave's frontier lives in two LDS stacks, one of nodes and
+// one of leaves, each entry tagged with its task's slot.  Every step a lane
+// takes one node (its four child box tests; hit internal children join the
+// node stack, hit leaves the leaf stack) and one leaf (the exact test; a hit
+// is appended to the ray's list with an atomic slot), whatever task they
+// belong to, as one iteration of the per-lane walk does for its own ray.  So
+// a task with a large subtree is walked 64 entries at a time instead of by one
+// lane, and the round's length follows the tree depth and the wave's total
+// work rather than its longest walk.  An entry that does not fit its stack
+// becomes a task of the next pooled round; in the last round it marks its ray
+// for re-traversal.
 constexpr int kPoolTake = GRAY_POOL_TAKE;
 constexpr int kPoolStack = GRAY_POOL_STACK;
 static_assert(kPoolTake >= 1 && kPoolTake <= GRAY_WAVE, "pool take: 1 to 64 tasks");
-static_assert(kPoolStack >= GRAY_WAVE, "pool stack: at least one entry per lane");
+static_assert(kPoolStack >= 4 * GRAY_WAVE, "pool stack: room for one step's pushes");
+
+// The ray of task slot `slot` from the pool's LDS table ([component][slot]).
+__device__ __forceinline__ LaneRay pool_ray(const float *s_ray, int slot, float one) {
+    LaneRay q;
+    q.ox = s_ray[0 * kPoolTake + slot];
+    q.oy = s_ray[1 * kPoolTake + slot];
+    q.oz = s_ray[2 * kPoolTake + slot];
+    q.dx = s_ray[3 * kPoolTake + slot];
+    q.dy = s_ray[4 * kPoolTake + slot];
+    q.dz = s_ray[5 * kPoolTake + slot];
+    q.ix = s_ray[6 * kPoolTake + slot];
+    q.iy = s_ray[7 * kPoolTake + slot];
+    q.iz = s_ray[8 * kPoolTake + slot];
+    q.one = one;
+    return q;
+}
 
 __global__ __launch_bounds__(GRAY_WAVE) void k_collect_pool(TraceArgs A, const GeoRecord *__restrict__ geo,
                                                             const Bvh4Node *__restrict__ wide) {
-    __shared__ uint32_t s_ref[kPoolStack];
-    __shared__ uint8_t s_slot[kPoolStack];
+    __shared__ uint32_t s_node[kPoolStack], s_leaf[kPoolStack];
+    __shared__ uint8_t s_nslot[kPoolStack], s_lslot[kPoolStack];
     __shared__ float s_ray[9 * kPoolTake];  // [component][slot]: origin, direction, inverse direction
     __shared__ uint32_t s_g[kPoolTake];
     const int lane = threadIdx.x;
@@ -789,72 +822,102 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_pool(TraceArgs A, const G
             s_ray[8 * kPoolTake + lane] = r.iz;
             s_g[lane] = g;
         }
-        const uint64_t has = __ballot(start != GRAY_EMPTY);
-        if (start != GRAY_EMPTY) {
-            const int o = __popcll(has & below);
-            s_ref[o] = start;
-            s_slot[o] = (uint8_t)lane;
+        const bool sn0 = start != GRAY_EMPTY && !(start & GRAY_LEAF);
+        const bool sl0 = start != GRAY_EMPTY && (start & GRAY_LEAF);
+        const uint64_t hn = __ballot(sn0), hl = __ballot(sl0);
+        if (sn0) {
+            const int o = __popcll(hn & below);
+            s_node[o] = start;
+            s_nslot[o] = (uint8_t)lane;
         }
-        int sp = __popcll(has);
+        if (sl0) {
+            const int o = __popcll(hl & below);
+            s_leaf[o] = start & ~GRAY_LEAF;
+            s_lslot[o] = (uint8_t)lane;
+        }
+        int sn = __popcll(hn), sl = __popcll(hl);
         __syncthreads();
-        while (sp > 0) {
-            // pop the top entries, one per lane
-            const int n = min(sp, GRAY_WAVE);
-            sp -= n;
-            uint32_t ref = GRAY_EMPTY;
-            int slot = 0;
-            if (lane < n) {
-                ref = s_ref[sp + lane];
-                slot = s_slot[sp + lane];
+        while (sn + sl > 0) {
+            // take the top entry of each stack, one of each per lane
+            const int nn = min(sn, GRAY_WAVE), nl = min(sl, GRAY_WAVE);
+            sn -= nn;
+            sl -= nl;
+            uint32_t node = GRAY_EMPTY, leaf = GRAY_EMPTY;
+            int nslot = 0, lslot = 0;
+            if (lane < nn) {
+                node = s_node[sn + lane];
+                nslot = s_nslot[sn + lane];
             }
+            if (lane < nl) {
+                leaf = s_leaf[sl + lane];
+                lslot = s_lslot[sl + lane];
+            }
+            // both records requested before either is used
+            Node4V nd;
+            GeoV gv;
+            if (node != GRAY_EMPTY) nd = load_node4(wide, node);
+            if (leaf != GRAY_EMPTY) gv = load_geo(geo, leaf);
             uint32_t ch[4] = {GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY};
-            uint32_t g = 0;
-            if (ref != GRAY_EMPTY) {
-                LaneRay q;
-                q.ox = s_ray[0 * kPoolTake + slot];
-                q.oy = s_ray[1 * kPoolTake + slot];
-                q.oz = s_ray[2 * kPoolTake + slot];
-                q.dx = s_ray[3 * kPoolTake + slot];
-                q.dy = s_ray[4 * kPoolTake + slot];
-                q.dz = s_ray[5 * kPoolTake + slot];
-                q.ix = s_ray[6 * kPoolTake + slot];
-                q.iy = s_ray[7 * kPoolTake + slot];
-                q.iz = s_ray[8 * kPoolTake + slot];
-                q.one = one;
-                g = s_g[slot];
-                if (ref & GRAY_LEAF) {
-                    const uint32_t gid = ref & ~GRAY_LEAF;
-                    const GeoV gv = load_geo(geo, gid);
-                    int cnt = 0;
-                    bool ok = true;
-                    collect_leaf<true>(A, gv, gid, q, g, cap, athr, p, cnt, ok);
-                    if (!ok) atomicExch(A.list_n + g, kRayDead);
-                } else {
-                    const Node4V nd = load_node4(wide, ref);
-                    ch[0] = child_hit<0>(q, nd) ? nd.ch.x : GRAY_EMPTY;
-                    ch[1] = child_hit<1>(q, nd) ? nd.ch.y : GRAY_EMPTY;
-                    ch[2] = child_hit<2>(q, nd) ? nd.ch.z : GRAY_EMPTY;
-                    ch[3] = child_hit<3>(q, nd) ? nd.ch.w : GRAY_EMPTY;
-                }
+            uint32_t ng = 0;
+            if (node != GRAY_EMPTY) {
+                const LaneRay q = pool_ray(s_ray, nslot, one);
+                ng = s_g[nslot];
+                ch[0] = child_hit<0>(q, nd) ? nd.ch.x : GRAY_EMPTY;
+                ch[1] = child_hit<1>(q, nd) ? nd.ch.y : GRAY_EMPTY;
+                ch[2] = child_hit<2>(q, nd) ? nd.ch.z : GRAY_EMPTY;
+                ch[3] = child_hit<3>(q, nd) ? nd.ch.w : GRAY_EMPTY;
             }
-            // every lane has read its entry before any push overwrites it
+            if (leaf != GRAY_EMPTY) {
+                const LaneRay q = pool_ray(s_ray, lslot, one);
+                const uint32_t lg = s_g[lslot];
+                int cnt = 0;
+                bool ok = true;
+                collect_leaf<true>(A, gv, leaf, q, lg, cap, athr, p, cnt, ok);
+                if (!ok) atomicExch(A.list_n + lg, kRayDead);
+            }
+            // every lane has read its entries before any push overwrites them
             __syncthreads();
+            // an entry that does not fit becomes a task of the next round
+            // (A.budget >= 0) or, in the last round, sends its ray to
+            // re-traversal
             bool lost = false;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const uint64_t m = __ballot(ch[k] != GRAY_EMPTY);
-                if (ch[k] != GRAY_EMPTY) {
-                    const int o = sp + __popcll(m & below);
-                    if (o < kPoolStack) {
-                        s_ref[o] = ch[k];
-                        s_slot[o] = (uint8_t)slot;
-                    } else {
-                        lost = true;
+            auto spill = [&](uint32_t ref) {
+                if (A.budget >= 0) {
+                    const int o = atomicAdd(A.task_count_out, 1);
+                    if (o < A.task_cap) {
+                        A.task_out[o] = make_uint2(ng, ref);
+                        return;
                     }
                 }
-                sp = min(sp + __popcll(m), kPoolStack);
+                lost = true;
+            };
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool isn = ch[k] != GRAY_EMPTY && !(ch[k] & GRAY_LEAF);
+                const bool isl = ch[k] != GRAY_EMPTY && (ch[k] & GRAY_LEAF);
+                const uint64_t mn = __ballot(isn), ml = __ballot(isl);
+                if (isn) {
+                    const int o = sn + __popcll(mn & below);
+                    if (o < kPoolStack) {
+                        s_node[o] = ch[k];
+                        s_nslot[o] = (uint8_t)nslot;
+                    } else {
+                        spill(ch[k]);
+                    }
+                }
+                if (isl) {
+                    const int o = sl + __popcll(ml & below);
+                    if (o < kPoolStack) {
+                        s_leaf[o] = ch[k] & ~GRAY_LEAF;
+                        s_lslot[o] = (uint8_t)nslot;
+                    } else {
+                        spill(ch[k]);
+                    }
+                }
+                sn = min(sn + __popcll(mn), kPoolStack);
+                sl = min(sl + __popcll(ml), kPoolStack);
             }
-            if (lost) atomicExch(A.list_n + g, kRayDead);
+            if (lost) atomicExch(A.list_n + ng, kRayDead);
             __syncthreads();
         }
     }
@@ -2541,7 +2604,8 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
                 hipLaunchKernelGGL(k_collect<kDetStack>, grid, block, 0, s, A, A.geo, cnodes);
             GRAY_HIP(hipGetLastError());
             if ((rc = mark(1))) return rc;
-            // per-lane budgeted rounds, then the pooled round that finishes
+            // per-lane budgeted rounds, then pooled rounds; only the last may
+            // not hand work on
             for (int round = 0; tasks && round < kTaskRounds; ++round) {
                 A.task_in = ctx->tasks[round & 1];
                 A.task_count_in = count + round;
@@ -2549,11 +2613,19 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
                 A.task_out = ctx->tasks[(round + 1) & 1];
                 A.task_count_out = count + round + 1;
                 A.budget = args->collect_budget > 0 ? args->collect_budget : GRAY_TASK_BUDGET;
-                if (round + 1 < kTaskRounds)
+                if (round < kLaneRounds) {
                     hipLaunchKernelGGL(k_collect_tasks<kCollectStack>, dim3(kTaskWaves), block, 0, s, A, A.geo,
                                        cnodes);
-                else
+                } else if (round == kLaneRounds) {
+                    if (round + 1 == kTaskRounds) A.budget = -1;
                     hipLaunchKernelGGL(k_collect_pool, dim3(kTaskWaves), block, 0, s, A, A.geo, cnodes);
+                } else {
+                    // later pooled rounds take only what an overflowing stack
+                    // handed on (usually nothing): a small grid, so that an
+                    // empty round costs few atomics on its head
+                    if (round + 1 == kTaskRounds) A.budget = -1;
+                    hipLaunchKernelGGL(k_collect_pool, dim3(kPoolSpillWaves), block, 0, s, A, A.geo, cnodes);
+                }
                 GRAY_HIP(hipGetLastError());
             }
             if ((rc = mark(2))) return rc;

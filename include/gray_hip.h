@@ -326,10 +326,10 @@ int gray_sh_backward(gray_ctx *ctx, const gray_params *params, const gray_knobs 
  *      boxes, short stack in LDS, leaves queued in LDS), runs the exact
  *      ray-ellipsoid test (shaders.cu:22-47) and appends every hit to the
  *      ray's list in the context (16 B per hit, slot-major); after a budget
- *      of iterations the rest of a walk is handed on as {ray, subtree} tasks
- *      that persistent task rounds take one per lane (budgeted again); the
- *      last round pools the tasks a wave takes into one LDS stack that its
- *      64 lanes drain together, so a long walk is split across the wave;
+ *      of iterations the rest of a walk is handed on as {ray, subtree} tasks;
+ *      a persistent pooled round gives each wave 32 tasks whose frontier
+ *      (node and leaf stacks in LDS) its 64 lanes drain together, one node
+ *      and one leaf per lane and step, so long walks are split across lanes;
  *   2. composite: one lane per ray scans its list with a 16-entry register
  *      k-buffer, in the reference's batches of 32 beyond tmin (sub-rounds of
  *      16), front to back, stopping at T < t_threshold; lists longer than
