@@ -410,6 +410,14 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 #ifndef GRAY_POOL_ROUNDS
 #define GRAY_POOL_ROUNDS 2
 #endif
+// Nodes and leaves each lane takes per step of a pooled round (two of each:
+// two independent records in flight per lane and kind).
+#ifndef GRAY_POOL_NODES
+#define GRAY_POOL_NODES 2
+#endif
+#ifndef GRAY_POOL_LEAVES
+#define GRAY_POOL_LEAVES 2
+#endif
 // Tasks a wave of a pooled round takes at a time, and its LDS stacks.
 #ifndef GRAY_POOL_TAKE
 #define GRAY_POOL_TAKE 32
@@ -748,10 +756,11 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const 
 // Pooled task round: a wave takes kPoolTake tasks and its 64 lanes share
 // their work.  The wave's frontier lives in two LDS stacks, one of nodes and
 // one of leaves, each entry tagged with its task's slot.  Every step a lane
-// takes one node (its four child box tests; hit internal children join the
-// node stack, hit leaves the leaf stack) and one leaf (the exact test; a hit
-// is appended to the ray's list with an atomic slot), whatever task they
-// belong to, as one iteration of the per-lane walk does for its own ray.  So
+// takes kPoolNodes nodes (four child box tests each; hit internal children
+// join the node stack, hit leaves the leaf stack) and kPoolLeaves leaves (the
+// exact test; a hit is appended to the ray's list with an atomic slot),
+// whatever task they belong to, as iterations of the per-lane walk do for its
+// own ray.  So
 // a task with a large subtree is walked 64 entries at a time instead of by one
 // lane, and the round's length follows the tree depth and the wave's total
 // work rather than its longest walk.  An entry that does not fit its stack
@@ -759,8 +768,10 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const 
 // for re-traversal.
 constexpr int kPoolTake = GRAY_POOL_TAKE;
 constexpr int kPoolStack = GRAY_POOL_STACK;
+constexpr int kPoolNodes = GRAY_POOL_NODES;
+constexpr int kPoolLeaves = GRAY_POOL_LEAVES;
 static_assert(kPoolTake >= 1 && kPoolTake <= GRAY_WAVE, "pool take: 1 to 64 tasks");
-static_assert(kPoolStack >= 4 * GRAY_WAVE, "pool stack: room for one step's pushes");
+static_assert(kPoolStack >= 4 * GRAY_POOL_NODES * GRAY_WAVE, "pool stack: room for one step's pushes");
 
 // The ray of task slot `slot` from the pool's LDS table ([component][slot]).
 __device__ __forceinline__ LaneRay pool_ray(const float *s_ray, int slot, float one) {
@@ -838,41 +849,64 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_pool(TraceArgs A, const G
         int sn = __popcll(hn), sl = __popcll(hl);
         __syncthreads();
         while (sn + sl > 0) {
-            // take the top entry of each stack, one of each per lane
-            const int nn = min(sn, GRAY_WAVE), nl = min(sl, GRAY_WAVE);
+            // take the top entries of each stack: kPoolNodes nodes and
+            // kPoolLeaves leaves per lane
+            constexpr int PN = kPoolNodes, PL = kPoolLeaves;
+            const int nn = min(sn, PN * GRAY_WAVE), nl = min(sl, PL * GRAY_WAVE);
             sn -= nn;
             sl -= nl;
-            uint32_t node = GRAY_EMPTY, leaf = GRAY_EMPTY;
-            int nslot = 0, lslot = 0;
-            if (lane < nn) {
-                node = s_node[sn + lane];
-                nslot = s_nslot[sn + lane];
+            uint32_t node[PN], leaf[PL];
+            int nslot[PN], lslot[PL];
+#pragma unroll
+            for (int j = 0; j < PN; ++j) {
+                node[j] = GRAY_EMPTY;
+                nslot[j] = 0;
+                if (lane + j * GRAY_WAVE < nn) {
+                    node[j] = s_node[sn + lane + j * GRAY_WAVE];
+                    nslot[j] = s_nslot[sn + lane + j * GRAY_WAVE];
+                }
             }
-            if (lane < nl) {
-                leaf = s_leaf[sl + lane];
-                lslot = s_lslot[sl + lane];
+#pragma unroll
+            for (int j = 0; j < PL; ++j) {
+                leaf[j] = GRAY_EMPTY;
+                lslot[j] = 0;
+                if (lane + j * GRAY_WAVE < nl) {
+                    leaf[j] = s_leaf[sl + lane + j * GRAY_WAVE];
+                    lslot[j] = s_lslot[sl + lane + j * GRAY_WAVE];
+                }
             }
-            // both records requested before either is used
-            Node4V nd;
-            GeoV gv;
-            if (node != GRAY_EMPTY) nd = load_node4(wide, node);
-            if (leaf != GRAY_EMPTY) gv = load_geo(geo, leaf);
-            uint32_t ch[4] = {GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY, GRAY_EMPTY};
-            uint32_t ng = 0;
-            if (node != GRAY_EMPTY) {
-                const LaneRay q = pool_ray(s_ray, nslot, one);
-                ng = s_g[nslot];
-                ch[0] = child_hit<0>(q, nd) ? nd.ch.x : GRAY_EMPTY;
-                ch[1] = child_hit<1>(q, nd) ? nd.ch.y : GRAY_EMPTY;
-                ch[2] = child_hit<2>(q, nd) ? nd.ch.z : GRAY_EMPTY;
-                ch[3] = child_hit<3>(q, nd) ? nd.ch.w : GRAY_EMPTY;
+            // every record requested before any is used
+            Node4V nd[PN];
+            GeoV gv[PL];
+#pragma unroll
+            for (int j = 0; j < PN; ++j)
+                if (node[j] != GRAY_EMPTY) nd[j] = load_node4(wide, node[j]);
+#pragma unroll
+            for (int j = 0; j < PL; ++j)
+                if (leaf[j] != GRAY_EMPTY) gv[j] = load_geo(geo, leaf[j]);
+            uint32_t ch[PN][4];
+            uint32_t ng[PN];
+#pragma unroll
+            for (int j = 0; j < PN; ++j) {
+                ch[j][0] = ch[j][1] = ch[j][2] = ch[j][3] = GRAY_EMPTY;
+                ng[j] = 0;
+                if (node[j] != GRAY_EMPTY) {
+                    const LaneRay q = pool_ray(s_ray, nslot[j], one);
+                    ng[j] = s_g[nslot[j]];
+                    ch[j][0] = child_hit<0>(q, nd[j]) ? nd[j].ch.x : GRAY_EMPTY;
+                    ch[j][1] = child_hit<1>(q, nd[j]) ? nd[j].ch.y : GRAY_EMPTY;
+                    ch[j][2] = child_hit<2>(q, nd[j]) ? nd[j].ch.z : GRAY_EMPTY;
+                    ch[j][3] = child_hit<3>(q, nd[j]) ? nd[j].ch.w : GRAY_EMPTY;
+                }
             }
-            if (leaf != GRAY_EMPTY) {
-                const LaneRay q = pool_ray(s_ray, lslot, one);
-                const uint32_t lg = s_g[lslot];
+#pragma unroll
+            for (int j = 0; j < PL; ++j) {
+                if (leaf[j] == GRAY_EMPTY) continue;
+                const LaneRay q = pool_ray(s_ray, lslot[j], one);
+                const uint32_t lg = s_g[lslot[j]];
                 int cnt = 0;
                 bool ok = true;
-                collect_leaf<true>(A, gv, leaf, q, lg, cap, athr, p, cnt, ok);
+                collect_leaf<true>(A, gv[j], leaf[j], q, lg, cap, athr, p, cnt, ok);
                 if (!ok) atomicExch(A.list_n + lg, kRayDead);
             }
             // every lane has read its entries before any push overwrites them
@@ -880,44 +914,48 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_pool(TraceArgs A, const G
             // an entry that does not fit becomes a task of the next round
             // (A.budget >= 0) or, in the last round, sends its ray to
             // re-traversal
-            bool lost = false;
-            auto spill = [&](uint32_t ref) {
-                if (A.budget >= 0) {
-                    const int o = atomicAdd(A.task_count_out, 1);
-                    if (o < A.task_cap) {
-                        A.task_out[o] = make_uint2(ng, ref);
-                        return;
-                    }
-                }
-                lost = true;
-            };
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const bool isn = ch[k] != GRAY_EMPTY && !(ch[k] & GRAY_LEAF);
-                const bool isl = ch[k] != GRAY_EMPTY && (ch[k] & GRAY_LEAF);
-                const uint64_t mn = __ballot(isn), ml = __ballot(isl);
-                if (isn) {
-                    const int o = sn + __popcll(mn & below);
-                    if (o < kPoolStack) {
-                        s_node[o] = ch[k];
-                        s_nslot[o] = (uint8_t)nslot;
-                    } else {
-                        spill(ch[k]);
+            for (int j = 0; j < PN; ++j) {
+                bool lost = false;
+                auto spill = [&](uint32_t ref) {
+                    if (A.budget >= 0) {
+                        const int o = atomicAdd(A.task_count_out, 1);
+                        if (o < A.task_cap) {
+                            A.task_out[o] = make_uint2(ng[j], ref);
+                            return;
+                        }
                     }
-                }
-                if (isl) {
-                    const int o = sl + __popcll(ml & below);
-                    if (o < kPoolStack) {
-                        s_leaf[o] = ch[k] & ~GRAY_LEAF;
-                        s_lslot[o] = (uint8_t)nslot;
-                    } else {
-                        spill(ch[k]);
+                    lost = true;
+                };
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const uint32_t c = ch[j][k];
+                    const bool isn = c != GRAY_EMPTY && !(c & GRAY_LEAF);
+                    const bool isl = c != GRAY_EMPTY && (c & GRAY_LEAF);
+                    const uint64_t mn = __ballot(isn), ml = __ballot(isl);
+                    if (isn) {
+                        const int o = sn + __popcll(mn & below);
+                        if (o < kPoolStack) {
+                            s_node[o] = c;
+                            s_nslot[o] = (uint8_t)nslot[j];
+                        } else {
+                            spill(c);
+                        }
                     }
+                    if (isl) {
+                        const int o = sl + __popcll(ml & below);
+                        if (o < kPoolStack) {
+                            s_leaf[o] = c & ~GRAY_LEAF;
+                            s_lslot[o] = (uint8_t)nslot[j];
+                        } else {
+                            spill(c);
+                        }
+                    }
+                    sn = min(sn + __popcll(mn), kPoolStack);
+                    sl = min(sl + __popcll(ml), kPoolStack);
                 }
-                sn = min(sn + __popcll(mn), kPoolStack);
-                sl = min(sl + __popcll(ml), kPoolStack);
+                if (lost) atomicExch(A.list_n + ng[j], kRayDead);
             }
-            if (lost) atomicExch(A.list_n + ng, kRayDead);
             __syncthreads();
         }
     }
