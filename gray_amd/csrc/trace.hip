@@ -1231,7 +1231,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_forward(TraceArgs A, const 
 // ---------------------------------------------------------------------------
 // Forward from the collected lists (lanes whose list is complete)
 // ---------------------------------------------------------------------------
-constexpr uint32_t kListMark = 0x80000000u;  // list id flag: entry was composited
+constexpr uint32_t kListMark = 0x80000000u;  // list id flag: entry was composited (entries >= kMaskEntries)
+constexpr int kMaskEntries = 128;             // entries whose flag is a register bit instead
 constexpr int kListBatch = 8;                 // list entries loaded per step of a scan
 constexpr int kWavePer = 8;                   // list entries per lane of the wave-per-ray kernel
 constexpr int kWaveListMax = GRAY_WAVE * kWavePer;
@@ -1281,9 +1282,11 @@ struct RegKB {
 // source: a round scans the lane's list (loads issued kListBatch entries at a
 // time) into a register k-buffer keyed by (distance, id) with the list slot
 // as payload, then composites front to back and flags every composited
-// entry.  After the last round one more scan sums the unflagged entries,
-// which are exactly the skip set (hits the k-buffer never handed to
-// compositing, hits beyond zfar, hits after termination).
+// entry: a bit of a 128-bit register mask for the first 128 entries, the id's
+// top bit in the list record for later ones.  After the last round one more
+// scan sums the unflagged entries, which are exactly the skip set (hits the
+// k-buffer never handed to compositing, hits beyond zfar, hits after
+// termination).
 __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, const float *__restrict__ rgb0) {
     int vi, tx, ty, tile;
     if (!block_tile(A, vi, tx, ty, &tile)) return;
@@ -1326,6 +1329,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
     float T = 1.0f, Tf = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f;
     float tmin = znear;
     uint32_t ncomp = 0;
+    uint64_t mask0 = 0, mask1 = 0;  // composited entries 0-63, 64-127
     const bool traced = active && tmin < zfar;
     bool alive = traced;
     // round cursor, as in the re-traversal forward
@@ -1390,7 +1394,14 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
                 V.hit_ids[(size_t)ncomp * R + pix] = gid;
                 V.hit_T[(size_t)ncomp * R + pix] = T;
             }
-            reinterpret_cast<uint32_t *>(L + e)[1] = gid | kListMark;
+            const uint32_t sl = kb.slot[k];
+            if (sl < (uint32_t)kMaskEntries) {
+                const uint64_t bit = 1ull << (sl & 63u);
+                mask0 |= sl < 64u ? bit : 0ull;
+                mask1 |= sl < 64u ? 0ull : bit;
+            } else {
+                reinterpret_cast<uint32_t *>(L + e)[1] = gid | kListMark;
+            }
             T = T * (1.0f - a);
             Tf = Tf * (1.0f - a32);
             ++ncomp;
@@ -1440,7 +1451,10 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
 #pragma unroll
         for (int j = 0; j < kListBatch; ++j) {
             const uint32_t id = __float_as_uint(sx[j].x);
-            w[j] = (id & kListMark) ? 0.0f : sx[j].y;  // fp32 alpha; its fp16 rounding enters the sums below
+            const int i = i0 + j;
+            const uint64_t mw = i < 64 ? mask0 : mask1;
+            const bool marked = i < kMaskEntries ? ((mw >> (i & 63)) & 1ull) != 0 : (id & kListMark) != 0;
+            w[j] = marked ? 0.0f : sx[j].y;  // fp32 alpha; its fp16 rounding enters the sums below
             const uint32_t gid = id & ~kListMark;
             c[j][0] = rgb[3 * gid + 0];
             c[j][1] = rgb[3 * gid + 1];
