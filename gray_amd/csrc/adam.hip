@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kThreads) void k_adam(AdamArgs A) {
     if (mode == 2) {
         for (int j = 0; j < kVecPerThread * 4; ++j) {
             const int64_t i = base + (int64_t)j * kThreads + threadIdx.x;
-            if (i < n) G[i] = G[i] * 0.0f;
+            if (i < n) G[i] = 0.0f;  // adam.cu:35: the channel gradient is set to +0
         }
         return;
     }
@@ -122,6 +122,7 @@ __global__ __launch_bounds__(kThreads) void k_adam(AdamArgs A) {
             float4 g = load_stream(G + i);
             float4 m = load_stream(M + i);
             float4 v = load_stream(V + i);
+            const float4 g0 = g;
             adam1(p.x, m.x, v.x, g.x, lr, b1, b2, bc1, bc2, eps, zero);
             adam1(p.y, m.y, v.y, g.y, lr, b1, b2, bc1, bc2, eps, zero);
             adam1(p.z, m.z, v.z, g.z, lr, b1, b2, bc1, bc2, eps, zero);
@@ -129,7 +130,14 @@ __global__ __launch_bounds__(kThreads) void k_adam(AdamArgs A) {
             *reinterpret_cast<float4 *>(P + i) = p;
             store_stream(m, M + i);
             store_stream(v, V + i);
-            if (zero) store_stream(g, G + i);
+            // zeroing leaves a gradient that is already +-0 bit for bit as it
+            // is (x * 0 keeps the sign of a zero): the store is skipped for the
+            // Gaussians no ray reached, about two thirds of a training frame's
+            const bool same = __float_as_uint(g.x) == __float_as_uint(g0.x) &&
+                              __float_as_uint(g.y) == __float_as_uint(g0.y) &&
+                              __float_as_uint(g.z) == __float_as_uint(g0.z) &&
+                              __float_as_uint(g.w) == __float_as_uint(g0.w);
+            if (zero && !same) store_stream(g, G + i);
         } else {
             for (int64_t k = i; k < n; ++k) {
                 float p = P[k], g = G[k], m = M[k], v = V[k];

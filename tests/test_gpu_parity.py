@@ -296,6 +296,10 @@ def test_adam_matches_oracle():
     host = {}
     for name in g.PARAMS:
         grad = torch.randn(getattr(g, name).shape, generator=gen)
+        # Gaussians no ray reached have +0 gradients (the kernel skips storing
+        # them again), and -0 must stay -0 as the reference's g *= 0 leaves it
+        grad[100:300] = 0.0
+        grad[300:340] = -0.0
         getattr(g, "grad_" + name).copy_(grad)
         host[name] = [getattr(g, name).cpu().numpy().copy(), grad.numpy().copy(),
                       np.zeros(grad.shape, np.float32), np.zeros(grad.shape, np.float32)]
@@ -308,14 +312,17 @@ def test_adam_matches_oracle():
     for name in g.PARAMS:
         p, gr, m, v = host[name]
         if name == "channels":
-            # SH on: channels are not optimised, their gradient is zeroed (adam.cu:35-40)
-            assert np.all(getattr(g, "grad_" + name).cpu().numpy() == 0)
+            # SH on: channels are not optimised, their gradient is set to +0 (adam.cu:35-40)
+            gc = getattr(g, "grad_" + name).cpu().numpy()
+            assert np.all(gc == 0) and not np.signbit(gc).any()
             continue
         O.adam(p, gr, m, v, lrs[name], 0.9, 0.999, 1e-15, 1)
         assert np.abs(getattr(g, name).cpu().numpy() - p).max() < 1e-6, name
         assert np.abs(getattr(g, "first_moment_" + name).cpu().numpy() - m).max() < 1e-6
         assert np.abs(getattr(g, "second_moment_" + name).cpu().numpy() - v).max() < 1e-6
-        assert np.all(getattr(g, "grad_" + name).cpu().numpy() == 0)
+        ga = getattr(g, "grad_" + name).cpu().numpy()
+        assert np.all(ga == 0)
+        assert np.array_equal(np.signbit(ga), np.signbit(gr)), name  # g *= 0 keeps the sign (adam.cu:19-20)
     assert np.all(g.pruning_counter.cpu().numpy() == 1)
 
 
