@@ -390,23 +390,14 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 #ifndef GRAY_LEAF_QUEUE
 #define GRAY_LEAF_QUEUE 8
 #endif
-// Iterations a walk runs before it hands its remaining work on as tasks: the
-// per-tile pass (GRAY_COLLECT_BUDGET) and the optional per-lane task rounds
-// (GRAY_TASK_BUDGET).
+// Iterations the per-tile walk runs before it hands its remaining work on as
+// tasks.
 #ifndef GRAY_COLLECT_BUDGET
 #define GRAY_COLLECT_BUDGET 128
 #endif
-#ifndef GRAY_TASK_BUDGET
-#define GRAY_TASK_BUDGET 64
-#endif
-// Task rounds after the first walk: per-lane budgeted rounds
-// (k_collect_tasks; none by default), then pooled rounds (k_collect_pool).
-// A pooled round finishes every task it takes; entries that overflow its
-// stacks are handed to the next pooled round, and the last one sends their
-// rays to re-traversal instead.
-#ifndef GRAY_LANE_ROUNDS
-#define GRAY_LANE_ROUNDS 0
-#endif
+// Pooled task rounds after the first walk (k_collect_pool).  A round finishes
+// every task it takes; entries that overflow its stacks are handed to the
+// next round, and the last one sends their rays to re-traversal instead.
 #ifndef GRAY_POOL_ROUNDS
 #define GRAY_POOL_ROUNDS 2
 #endif
@@ -429,9 +420,8 @@ __device__ __forceinline__ int kb_insert(const KBuf &kb, int &count, float &last
 constexpr int kCollectStack = GRAY_COLLECT_STACK;
 constexpr int kCollectLeaves = GRAY_COLLECT_LEAVES;
 constexpr int kLeafQueue = GRAY_LEAF_QUEUE;
-constexpr int kLaneRounds = GRAY_LANE_ROUNDS;
-constexpr int kTaskRounds = GRAY_LANE_ROUNDS + GRAY_POOL_ROUNDS;
-static_assert(GRAY_LANE_ROUNDS >= 0 && GRAY_POOL_ROUNDS >= 1, "at least one pooled task round");
+constexpr int kTaskRounds = GRAY_POOL_ROUNDS;
+static_assert(kTaskRounds >= 1, "at least one pooled task round");
 #ifndef GRAY_TASK_WAVES
 #define GRAY_TASK_WAVES 4096
 #endif
@@ -511,14 +501,13 @@ __device__ __forceinline__ void collect_leaf(const TraceArgs &A, const GeoV &q, 
 // dense across the wave.  Node visits pause while the queue could not take
 // four leaves.
 //
-// Every hit goes to the ray's list: at slot cnt (the per-tile pass owns the
-// ray) or at a slot taken from list_n[g] with an atomic (tasks of one ray run
-// concurrently).  After `budget` iterations (< 0: none) a lane that still has
-// work appends it to the task buffer as {g, reference} pairs and stops, so no
-// wave runs longer than the budget however many boxes its rays cross.  ok
-// turns false when the stack, the list or the task buffer overflows: the ray
-// is then re-traversed instead.  Must be called by the whole wave.
-template <bool kTask, int kStack>
+// Every hit goes to the ray's list at slot cnt (the per-tile pass owns the
+// ray).  After `budget` iterations (< 0: none) a lane that still has work
+// appends it to the task buffer as {g, reference} pairs and stops, so no wave
+// runs longer than the budget however many boxes its rays cross.  ok turns
+// false when the stack, the list or the task buffer overflows: the ray is
+// then re-traversed instead.  Must be called by the whole wave.
+template <int kStack>
 __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord *__restrict__ geo,
                                                   const Bvh4Node *__restrict__ wide, const LaneRay &r, size_t g,
                                                   uint32_t start, int budget, uint32_t *__restrict__ s_stack,
@@ -601,7 +590,7 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
         for (int l = 0; l < NL; ++l) {
             if (pl[l] == GRAY_EMPTY) continue;
             ++leaves;
-            collect_leaf<kTask>(A, pg[l], pl[l], r, g, cap, athr, p, cnt, ok);
+            collect_leaf<false>(A, pg[l], pl[l], r, g, cap, athr, p, cnt, ok);
         }
         uint32_t nh = h;
         bool moved = false;
@@ -682,7 +671,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
     const size_t g = (size_t)tile * GRAY_WAVE + lane;
     int cnt = 0, visits = 0, leaves = 0;
     bool ok = true;
-    collect_walk<false, kStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
+    collect_walk<kStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
                                       A.budget, s_stack, s_leaf, lane, cnt, ok, visits, leaves);
     A.list_n[g] = ok ? cnt : -1;
     if (A.tile_stats) {
@@ -714,43 +703,6 @@ __device__ __forceinline__ const DevView &lane_pixel(const TraceArgs &A, uint32_
     x = (lt % V.tiles_x) * GRAY_TILE + (lane & 7);
     y = (lt / V.tiles_x) * GRAY_TILE + (lane >> 3);
     return V;
-}
-
-// A task round: persistent waves take the round's tasks 64 at a time (one per
-// lane) and walk each task's subtree for its ray.  The task count was fixed by
-// the previous kernel, so every wave leaves the loop once the shared head
-// passes it.
-template <int kStack>
-__global__ __launch_bounds__(GRAY_WAVE) void k_collect_tasks(TraceArgs A, const GeoRecord *__restrict__ geo,
-                                                             const Bvh4Node *__restrict__ nodes) {
-    __shared__ uint32_t s_stack[kStack * GRAY_WAVE];
-    __shared__ uint32_t s_leaf[kLeafQueue * GRAY_WAVE];
-    const int lane = threadIdx.x;
-    const int total = min(*A.task_count_in, A.task_cap);
-    for (;;) {
-        int base = 0;
-        if (lane == 0) base = atomicAdd(A.task_head, GRAY_WAVE);
-        base = __shfl(base, 0);
-        if (base >= total) break;
-        const int t = base + lane;
-        uint32_t start = GRAY_EMPTY;
-        uint32_t g = 0;
-        LaneRay r{};
-        if (t < total) {
-            const uint2 task = A.task_in[t];
-            if (task.x < (uint64_t)A.list_stride) {
-                g = task.x;
-                int x, y;
-                const DevView &V = lane_pixel(A, g, x, y);
-                if (x < V.width && y < V.height && A.list_n[g] >= 0 && make_ray(A.knobs, V, x, y, r)) start = task.y;
-            }
-        }
-        int cnt = 0, visits = 0, leaves = 0;
-        bool ok = true;
-        collect_walk<true, kStack>(A, geo, nodes, r, g, start, A.budget, s_stack, s_leaf, lane, cnt, ok, visits,
-                                  leaves);
-        if (!ok && start != GRAY_EMPTY) atomicExch(A.list_n + g, kRayDead);
-    }
 }
 
 // Pooled task round: a wave takes kPoolTake tasks and its 64 lanes share
@@ -2656,28 +2608,20 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
                 hipLaunchKernelGGL(k_collect<kDetStack>, grid, block, 0, s, A, A.geo, cnodes);
             GRAY_HIP(hipGetLastError());
             if ((rc = mark(1))) return rc;
-            // per-lane budgeted rounds, then pooled rounds; only the last may
-            // not hand work on
+            // pooled rounds; all but the last may hand work on (budget >= 0)
             for (int round = 0; tasks && round < kTaskRounds; ++round) {
                 A.task_in = ctx->tasks[round & 1];
                 A.task_count_in = count + round;
                 A.task_head = head + round;
                 A.task_out = ctx->tasks[(round + 1) & 1];
                 A.task_count_out = count + round + 1;
-                A.budget = args->collect_budget > 0 ? args->collect_budget : GRAY_TASK_BUDGET;
-                if (round < kLaneRounds) {
-                    hipLaunchKernelGGL(k_collect_tasks<kCollectStack>, dim3(kTaskWaves), block, 0, s, A, A.geo,
-                                       cnodes);
-                } else if (round == kLaneRounds) {
-                    if (round + 1 == kTaskRounds) A.budget = -1;
-                    hipLaunchKernelGGL(k_collect_pool, dim3(kTaskWaves), block, 0, s, A, A.geo, cnodes);
-                } else {
-                    // later pooled rounds take only what an overflowing stack
-                    // handed on (usually nothing): a small grid, so that an
-                    // empty round costs few atomics on its head
-                    if (round + 1 == kTaskRounds) A.budget = -1;
-                    hipLaunchKernelGGL(k_collect_pool, dim3(kPoolSpillWaves), block, 0, s, A, A.geo, cnodes);
-                }
+                A.budget = round + 1 < kTaskRounds ? 0 : -1;
+                // later rounds take only what an overflowing stack handed on
+                // (usually nothing): a small grid, so that an empty round costs
+                // few atomics on its head (atomics on one address serialise:
+                // 4096 waves taking nothing cost about 40 us)
+                hipLaunchKernelGGL(k_collect_pool, dim3(round == 0 ? kTaskWaves : kPoolSpillWaves), block, 0, s, A,
+                                   A.geo, cnodes);
                 GRAY_HIP(hipGetLastError());
             }
             if ((rc = mark(2))) return rc;
