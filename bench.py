@@ -4,9 +4,10 @@ Default workload (one MI355X): the 1M-Gaussian training step -- 1,000,000
 Gaussians on 20 random surfaces (log-scale N(-4.5, 0.4^2)), one 980x545
 pinhole camera (fov 60 deg) on a ring of radius 6; per step: forward render,
 L1 loss against a random target, backward, fused Adam and BVH refit.
-Prints one JSON line (metric/value/unit/...) with the dominant kernel's HBM
-roofline (HIP-event launch time, HBM traffic from rocprofv3 counters) and a
-bounded CPU-oracle baseline timed on the host cores.
+Prints one JSON line (metric/value/unit/...) with the HBM roofline of the
+dominant unit, the collect pass (HIP-event time of its launches, HBM traffic
+from rocprofv3 counters), and a bounded CPU-oracle baseline timed on the host
+cores.
 
   python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-traffic] [--no-extra]
 
@@ -41,10 +42,14 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "train-step ms & rays/s at 1M Gaussians; p50/p99 frame latency vs plain launch"
-DOMINANT_KERNEL = "k_collect"
-# algorithmic bytes of the collect kernel per Gaussian: the parameters its
-# exact hit tests depend on (mean 12, rotation 16, scale 12, opacity 4), read
-# once; hit lists, BVH nodes and the snapshot are implementation traffic
+# the dominant unit: the collect pass, i.e. the per-tile walk (k_collect) and
+# the pooled task rounds that finish its walks (k_collect_pool, two launches)
+DOMINANT_KERNEL = "collect pass (k_collect + k_collect_pool)"
+PASS_KERNELS = ("k_collect<", "k_collect(", "k_collect_pool(")
+PASS_FIRST = ("k_collect<", "k_collect(")  # one launch per pass
+# algorithmic bytes of the collect pass per Gaussian: the parameters its exact
+# hit tests depend on (mean 12, rotation 16, scale 12, opacity 4), read once;
+# hit lists, BVH nodes and the snapshot are implementation traffic
 COLLECT_BYTES_PER_GAUSSIAN = 44
 # algorithmic bytes of a whole training step (params, grads, Adam state and
 # images, each moved once): 2640 per Gaussian + 60 per ray
@@ -184,25 +189,29 @@ def traffic_probe(args) -> None:
 
 
 def _counter_means(csv_path: str, counter: str):
-    kern, cal = [], []
+    """(counter per collect pass: summed over the pass's launches, averaged
+    over the passes; counter per calibration launch)."""
+    total, passes, cal = 0.0, 0, []
     with open(csv_path) as f:
         for row in csv.DictReader(f):
             if row["Counter_Name"] != counter:
                 continue
             name = row["Kernel_Name"]
-            if f"{DOMINANT_KERNEL}(" in name or f"{DOMINANT_KERNEL}<" in name:
-                kern.append(float(row["Counter_Value"]))
+            if any(k in name for k in PASS_KERNELS):
+                total += float(row["Counter_Value"])
+                passes += any(k in name for k in PASS_FIRST)
             elif "neg_kernel" in name:
                 cal.append(float(row["Counter_Value"]))
-    if not kern or not cal:
+    if not passes or not cal:
         return None
-    return sum(kern) / len(kern), sum(cal) / len(cal)
+    return total / passes, sum(cal) / len(cal)
 
 
 def measure_traffic(args):
-    """HBM bytes per launch of the dominant kernel: one rocprofv3 --pmc run
-    per counter (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by
-    what the same counter reads for the calibration kernel of known size."""
+    """HBM bytes per collect pass: one rocprofv3 --pmc run per counter
+    (FETCH_SIZE, WRITE_SIZE, KiB per dispatch, summed over the pass's
+    launches), each scaled by what the same counter reads for the calibration
+    kernel of known size."""
     rocprof = shutil.which("rocprofv3")
     if not rocprof:
         return None, "rocprofv3 not found"
@@ -311,7 +320,8 @@ def run_train(args, traffic):
         stages.append(nm.forward_stage_ms())
     nm.set_device_timing(False)
     stage_ms = {k: sum(s[k] for s in stages) / len(stages) for k in stages[0]}
-    launch_ms = stage_ms["collect"]
+    # the collect pass: the per-tile walk and the pooled task rounds
+    launch_ms = stage_ms["collect"] + stage_ms["collect_tasks"]
     alg = COLLECT_BYTES_PER_GAUSSIAN * n
     achieved = alg / (launch_ms * 1e-3) / 1e9
     step_alg = STEP_BYTES_PER_GAUSSIAN * n + STEP_BYTES_PER_RAY * rays
