@@ -57,11 +57,18 @@ struct SnapF3 {
     float x, y, z;
 };
 
-__global__ void k_snapshot(int n, const float *__restrict__ mean, const float *__restrict__ rotation,
+constexpr int kSnapThreads = 256;
+
+__global__ __launch_bounds__(kSnapThreads) void k_snapshot(int n, const float *__restrict__ mean, const float *__restrict__ rotation,
                            const float *__restrict__ scale, const float *__restrict__ opacity, DevKnobs kn,
                            GeoRecord *__restrict__ geo, float *__restrict__ l2w, float *__restrict__ aabb) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n) return;
+    // the block's records are assembled in LDS and stored by whole lines (a
+    // thread's own 64-, 48- and 24-byte records would each cover parts of
+    // lines per store instruction): 56 -> 44 us at 1M Gaussians
+    __shared__ float4 s_geo[kSnapThreads * 4], s_l2w[kSnapThreads * 3];
+    __shared__ float2 s_box[kSnapThreads * 3];
+    const int g0 = blockIdx.x * kSnapThreads, t = threadIdx.x;
+    const int g = min(g0 + t, n - 1);  // a tail thread recomputes the last record; its copy is not stored
     const float athr = *kn.alpha_threshold, p = *kn.exp_power, gsf = *kn.gsf;
     // the parameters were last written by the optimizer's non-temporal
     // stores and are read once here: non-temporal loads keep them from
@@ -112,11 +119,14 @@ __global__ void k_snapshot(int n, const float *__restrict__ mean, const float *_
     rec.op = visible ? op : -1.0f;
     rec.clip = clip;
     rec.pad0 = rec.pad1 = 0.0f;
-    geo[g] = rec;
-    float4 *L = reinterpret_cast<float4 *>(l2w + 12 * (size_t)g);
-    L[0] = make_float4(A[0], A[1], A[2], A[3]);
-    L[1] = make_float4(A[4], A[5], A[6], A[7]);
-    L[2] = make_float4(A[8], 0.0f, 0.0f, 0.0f);
+    {
+        const float *rf = reinterpret_cast<const float *>(&rec);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s_geo[4 * t + k] = make_float4(rf[4 * k], rf[4 * k + 1], rf[4 * k + 2], rf[4 * k + 3]);
+        s_l2w[3 * t + 0] = make_float4(A[0], A[1], A[2], A[3]);
+        s_l2w[3 * t + 1] = make_float4(A[4], A[5], A[6], A[7]);
+        s_l2w[3 * t + 2] = make_float4(A[8], 0.0f, 0.0f, 0.0f);
+    }
 
     float box[6];
     if (visible) {
@@ -131,10 +141,17 @@ __global__ void k_snapshot(int n, const float *__restrict__ mean, const float *_
     } else {
         box_empty(box);
     }
-    float2 *ab = reinterpret_cast<float2 *>(aabb + 6 * (size_t)g);
-    ab[0] = make_float2(box[0], box[1]);
-    ab[1] = make_float2(box[2], box[3]);
-    ab[2] = make_float2(box[4], box[5]);
+    s_box[3 * t + 0] = make_float2(box[0], box[1]);
+    s_box[3 * t + 1] = make_float2(box[2], box[3]);
+    s_box[3 * t + 2] = make_float2(box[4], box[5]);
+    __syncthreads();
+    const int cnt = min(kSnapThreads, n - g0);
+    float4 *dg = reinterpret_cast<float4 *>(geo + g0);
+    float4 *dl = reinterpret_cast<float4 *>(l2w + 12 * (size_t)g0);
+    float2 *db = reinterpret_cast<float2 *>(aabb + 6 * (size_t)g0);
+    for (int i = t; i < 4 * cnt; i += kSnapThreads) dg[i] = s_geo[i];
+    for (int i = t; i < 3 * cnt; i += kSnapThreads) dl[i] = s_l2w[i];
+    for (int i = t; i < 3 * cnt; i += kSnapThreads) db[i] = s_box[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -905,7 +922,7 @@ extern "C" int gray_bvh_update(gray_ctx *ctx, const gray_params *params, const g
                 knobs->eps_forward_normalization, knobs->eps_scale_grad, knobs->background_channels,
                 knobs->render_depth, knobs->enable_sh, knobs->jitter_primary_rays, knobs->call_index,
                 knobs->needs_ray_output};
-    hipLaunchKernelGGL(k_snapshot, dim3(gray_div_up(n, 256)), dim3(256), 0, s, (int)n, params->mean,
+    hipLaunchKernelGGL(k_snapshot, dim3(gray_div_up(n, kSnapThreads)), dim3(kSnapThreads), 0, s, (int)n, params->mean,
                        params->rotation, params->scale, params->opacity, kn, ctx->geo, ctx->l2w, ctx->aabb);
     GRAY_HIP(hipGetLastError());
     if (rebuild) {
