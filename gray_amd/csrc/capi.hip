@@ -42,6 +42,9 @@ extern "C" int gray_ctx_create(gray_ctx **out) {
     if (e == hipSuccess) e = hipEventCreate(&ctx->ev1);
     for (hipEvent_t &fe : ctx->fev)
         if (e == hipSuccess) e = hipEventCreate(&fe);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         gray_check_hip(e, "ctx_create");
         gray_ctx_destroy(ctx);
@@ -66,6 +69,9 @@ extern "C" void gray_ctx_destroy(gray_ctx *ctx) {
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     for (hipEvent_t e : ctx->fev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     delete ctx;
 }
 

@@ -66,6 +66,11 @@ struct gray_ctx {
     bool timing = false;
     float last_build_ms = -1.0f;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // second stream of the forward (default priority): the wave-per-ray
+    // compositing runs on it beside the per-lane compositing, forked from and
+    // joined back into the caller's stream with these events
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // forward stage events (recorded when timing is on): collect, task rounds,
     // list compositing, re-traversal
     hipEvent_t fev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

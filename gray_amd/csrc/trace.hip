@@ -1237,6 +1237,28 @@ struct RegKB {
     }
 };
 
+// Long lists (wave_list_min < n <= kWaveListMax) go to the queue of the
+// wave-per-ray kernel: one queue slot per lane, taken with one atomic per
+// wave.  Rays outside the image or inactive have empty lists.
+__device__ __forceinline__ bool heavy_list(const TraceArgs &A, int n) {
+    return A.wave_list_min >= 0 && n > A.wave_list_min && n <= kWaveListMax;
+}
+
+__global__ __launch_bounds__(GRAY_WAVE) void k_heavy_queue(TraceArgs A) {
+    int vi, tx, ty, tile;
+    if (!block_tile(A, vi, tx, ty, &tile)) return;
+    const int lane = threadIdx.x;
+    const size_t lg = (size_t)tile * GRAY_WAVE + lane;
+    const bool heavy = heavy_list(A, A.list_n[lg]);
+    const uint64_t hb = __ballot(heavy);
+    if (!hb) return;
+    const int first = __ffsll((long long)hb) - 1;
+    int base = 0;
+    if (lane == first) base = atomicAdd(A.heavy_count, __popcll(hb));
+    base = __shfl(base, first);
+    if (heavy) A.heavy_rays[base + __popcll(hb & ((1ull << lane) - 1ull))] = (uint32_t)lg;
+}
+
 // Same batch semantics as the re-traversal forward, with the list as the hit
 // source: a round scans the lane's list (loads issued kListBatch entries at a
 // time) into a register k-buffer keyed by (distance, id) with the list slot
@@ -1261,18 +1283,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
     // returns at once unless some wave raised this flag
     if (__ballot(active && n < 0) && lane == 0) atomicOr(A.retrace_any, 1);
     active = active && n >= 0;
-    // long lists go to the wave-per-ray kernel (one queue slot per lane,
-    // taken with one atomic per wave)
-    const bool heavy = active && A.wave_list_min >= 0 && n > A.wave_list_min && n <= kWaveListMax;
-    const uint64_t hb = __ballot(heavy);
-    if (hb) {
-        const int first = __ffsll((long long)hb) - 1;
-        int base = 0;
-        if (lane == first) base = atomicAdd(A.heavy_count, __popcll(hb));
-        base = __shfl(base, first);
-        if (heavy) A.heavy_rays[base + __popcll(hb & ((1ull << lane) - 1ull))] = (uint32_t)lg;
-    }
-    active = active && !heavy;
+    // long lists are composited by the wave-per-ray kernel (queued by k_heavy_queue)
+    active = active && !heavy_list(A, n);
     if (!__ballot(active)) return;
 
     float4 *__restrict__ L = A.list_rec;
@@ -2638,12 +2650,21 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             A.heavy_rays = reinterpret_cast<uint32_t *>(ctx->tasks[0]);
             A.wave_list_min = args->wave_list_min == 0 ? GRAY_WAVE_LIST_MIN : args->wave_list_min;
             A.retrace_any = ctx->task_counters + 2 * kTaskRounds + 3;  // zeroed with the task counters
+            if (A.wave_list_min >= 0) {
+                // the long lists are queued first; their wave-per-ray kernel then
+                // runs on the side stream beside the per-lane kernel (both are
+                // latency-bound) and is joined back before the re-traversal
+                hipLaunchKernelGGL(k_heavy_queue, grid, block, 0, s, A);
+                GRAY_HIP(hipGetLastError());
+                GRAY_HIP(hipEventRecord(ctx->ev_fork, s));
+                GRAY_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+                hipLaunchKernelGGL(k_composite_wave, dim3(kWaveGrid), block, 0, ctx->side, A, A.views[0].rgb);
+                GRAY_HIP(hipGetLastError());
+                GRAY_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+            }
             hipLaunchKernelGGL(k_composite_lists, grid, block, 0, s, A, A.views[0].rgb);
             GRAY_HIP(hipGetLastError());
-            if (A.wave_list_min >= 0) {
-                hipLaunchKernelGGL(k_composite_wave, dim3(kWaveGrid), block, 0, s, A, A.views[0].rgb);
-                GRAY_HIP(hipGetLastError());
-            }
+            if (A.wave_list_min >= 0) GRAY_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
         } else {
             for (int i = 0; i < 3; ++i)
                 if ((rc = mark(i))) return rc;
