@@ -45,6 +45,7 @@ extern "C" int gray_ctx_create(gray_ctx **out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_sh, hipEventDisableTiming);
     if (e != hipSuccess) {
         gray_check_hip(e, "ctx_create");
         gray_ctx_destroy(ctx);
@@ -71,6 +72,7 @@ extern "C" void gray_ctx_destroy(gray_ctx *ctx) {
         if (e) (void)hipEventDestroy(e);
     if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_sh) (void)hipEventDestroy(ctx->ev_sh);
     if (ctx->side) (void)hipStreamDestroy(ctx->side);
     delete ctx;
 }

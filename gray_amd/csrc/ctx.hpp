@@ -71,6 +71,7 @@ struct gray_ctx {
     // joined back into the caller's stream with these events
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_sh = nullptr;  // the colours computed on the side stream (render args sh_params)
     // forward stage events (recorded when timing is on): collect, task rounds,
     // list compositing, re-traversal
     hipEvent_t fev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};

@@ -34,6 +34,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <vector>
+
 #include "ctx.hpp"
 
 namespace {
@@ -2572,6 +2574,33 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
     int rc = check_render(ctx, args);
     if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
+    // colours from the SH coefficients on the side stream, beside the collect
+    // pass (which does not read them); joined before the first kernel that does
+    bool sh_pending = false;
+    if (args->sh_params) {
+        std::vector<const float *> origins(args->num_views);
+        std::vector<float *> outs(args->num_views);
+        for (int i = 0; i < args->num_views; ++i) {
+            origins[i] = args->views[i].origin;
+            outs[i] = const_cast<float *>(args->views[i].rgb);
+            if (!origins[i] || !outs[i]) {
+                gray_set_error(GRAY_ERR_INVALID, "render_forward(sh_params): view %d lacks an origin or rgb", i);
+                return GRAY_ERR_INVALID;
+            }
+        }
+        GRAY_HIP(hipEventRecord(ctx->ev_fork, s));
+        GRAY_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        if ((rc = gray_sh_forward(ctx, args->sh_params, &args->knobs, args->num_views, origins.data(), outs.data(),
+                                  ctx->side)))
+            return rc;
+        GRAY_HIP(hipEventRecord(ctx->ev_sh, ctx->side));
+        sh_pending = true;
+    }
+    auto join_sh = [&]() -> int {
+        if (sh_pending) GRAY_HIP(hipStreamWaitEvent(s, ctx->ev_sh, 0));
+        sh_pending = false;
+        return GRAY_OK;
+    };
     int tiles_before = 0;
     for (int first = 0; first < args->num_views; first += GRAY_MAX_VIEWS) {
         const int nv = std::min(GRAY_MAX_VIEWS, args->num_views - first);
@@ -2644,6 +2673,7 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
                 GRAY_HIP(hipGetLastError());
             }
             if ((rc = mark(2))) return rc;
+            if ((rc = join_sh())) return rc;
             // the task buffers are free once the last round has run: the first
             // holds the queue of rays composited by a whole wave (one entry per lane)
             A.heavy_count = ctx->task_counters + 2 * kTaskRounds + 2;
@@ -2666,6 +2696,7 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             GRAY_HIP(hipGetLastError());
             if (A.wave_list_min >= 0) GRAY_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
         } else {
+            if ((rc = join_sh())) return rc;
             for (int i = 0; i < 3; ++i)
                 if ((rc = mark(i))) return rc;
         }
@@ -2675,7 +2706,7 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
         if ((rc = mark(4))) return rc;
         if (timed) ctx->fev_recorded = true;
     }
-    return GRAY_OK;
+    return join_sh();  // also when no view had a tile
 }
 
 extern "C" int gray_render_stage_ms(gray_ctx *ctx, float *ms, int32_t max_stages) {

@@ -200,11 +200,11 @@ class MultiViewRenderer:
         rebuild = rebuild or n != self._built_n
         L.check(L.LIB.gray_bvh_update(self.ctx.handle, C.byref(params), C.byref(knobs), 1 if rebuild else 0, stream))
         self._built_n = n
-        origins = (C.c_void_p * V)(*[self.cam_origin[i].data_ptr() for i in range(V)])
-        outs = (C.c_void_p * V)(*[self.rgb[i].data_ptr() for i in range(V)])
-        L.check(L.LIB.gray_sh_forward(self.ctx.handle, C.byref(params), C.byref(knobs), V, origins, outs, stream))
         views = self._views(V)
-        L.check(L.LIB.gray_render_forward(self.ctx.handle, C.byref(self._args(views, V, save)), stream))
+        args = self._args(views, V, save)
+        # the launch computes each view's SH colours itself, beside its collect pass
+        args.sh_params = C.cast(C.pointer(params), C.c_void_p)
+        L.check(L.LIB.gray_render_forward(self.ctx.handle, C.byref(args), stream))
         self.generation += 1
         return self.out
 

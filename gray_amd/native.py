@@ -452,26 +452,33 @@ class NativeRaytracer:
         save = torch.is_grad_enabled() and not self.inference_only
         if save and self._ellipsoids_on():
             raise RuntimeError("render_ellipsoids=True is only supported for no-grad display renders")
-        self._sh_forward()
         if not save and self._ellipsoids_on():
+            self._sh_forward()
             self._render_ellipsoids()
             self._forward_saved = False
             return
         view = self._view(save)
         tk = self.timer.start("trace_forward")
-        L.check(L.LIB.gray_render_forward(self.ctx.handle, C.byref(self._args(view, save)), self._stream()))
+        self._render(view, save)
         self.timer.stop(tk)
         self._forward_saved = save
+
+    def _render(self, view: L.View, save: bool) -> None:
+        """One forward launch; the launch computes the SH colours itself
+        (render args sh_params) beside its collect pass."""
+        args = self._args(view, save)
+        params = self.gaussian_data.params()
+        args.sh_params = C.cast(C.pointer(params), C.c_void_p)
+        L.check(L.LIB.gray_render_forward(self.ctx.handle, C.byref(args), self._stream()))
 
     def forward_pass_display(self) -> None:
         """Reference raytracer.h:104-114 (no metadata update, no backward state)."""
         self._require_bvh()
-        self._sh_forward()
         if self._ellipsoids_on():
+            self._sh_forward()
             self._render_ellipsoids()
             return
-        view = self._view(False)
-        L.check(L.LIB.gray_render_forward(self.ctx.handle, C.byref(self._args(view, False)), self._stream()))
+        self._render(self._view(False), False)
 
     def _ellipsoids_on(self) -> bool:
         """config.render_ellipsoids (host memory, no synchronisation): the

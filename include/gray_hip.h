@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 12
+#define GRAY_ABI_VERSION 13
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -149,6 +149,12 @@ typedef struct {
                                    gradients.  The backward then synchronizes the stream once per view and
                                    keeps 80 bytes per composited hit.  No counterpart in the reference
                                    (its gradients are summed with atomics); 0 = off */
+    const gray_params *sh_params; /* optional: the launch computes each view's colours from these parameters'
+                                   SH coefficients itself (as gray_sh_forward into views[].rgb from
+                                   views[].origin, with knobs), on the context's second stream while the
+                                   collect pass runs, and waits for them before compositing; the reference's
+                                   forward_pass evaluates SH inside the pass as well (raytracer.h:95-97).
+                                   NULL = views[].rgb already holds the colours */
 } gray_render_args;
 
 /* Gradient targets of the backward (reference gaussians.h:34-62). */
