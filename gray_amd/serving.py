@@ -200,10 +200,7 @@ class SceneServer:
                 v.out_channels = img.data_ptr()
                 v.out_depth = r.depth.data_ptr() if r.depth is not None else None
             sp = L.stream_ptr(slot.stream)
-            origins = (C.c_void_p * V)(*[views[i].origin for i in range(V)])
-            rgbs = (C.c_void_p * V)(*[views[i].rgb for i in range(V)])
             knobs = self.knobs(depth_on)
-            L.check(L.LIB.gray_sh_forward(slot.ctx.handle, C.byref(self.params), C.byref(knobs), V, origins, rgbs, sp))
             a = L.RenderArgs()
             a.knobs = knobs
             a.num_views = V
@@ -211,9 +208,15 @@ class SceneServer:
             a.save, a.hit_capacity, a.use_bvh, a.list_capacity = 0, 0, 1, self.list_capacity
             a.fp16_hits = 1 if s.fp16_hits else 0
             if ellipsoids:
+                origins = (C.c_void_p * V)(*[views[i].origin for i in range(V)])
+                rgbs = (C.c_void_p * V)(*[views[i].rgb for i in range(V)])
+                L.check(L.LIB.gray_sh_forward(slot.ctx.handle, C.byref(self.params), C.byref(knobs), V, origins, rgbs,
+                                              sp))
                 min_op = C.c_void_p(slot.cams[0].data_ptr() + 15 * 4)
                 L.check(L.LIB.gray_render_ellipsoids(slot.ctx.handle, C.byref(a), min_op, sp))
             else:
+                # the launch computes the colours itself, beside its collect pass
+                a.sh_params = C.cast(C.pointer(self.params), C.c_void_p)
                 L.check(L.LIB.gray_render_forward(slot.ctx.handle, C.byref(a), sp))
             slot.end = torch.cuda.Event(enable_timing=True)
             slot.end.record(slot.stream)

@@ -343,6 +343,9 @@ int gray_sh_backward(gray_ctx *ctx, const gray_params *params, const gray_knobs 
  *   3. rays whose list or tasks overflowed are re-traversed per batch by a
  *      k-buffer kernel (16 entries in LDS), as is every ray when
  *      list_capacity == 0.
+ * The context's second stream runs the SH colours (sh_params) beside the
+ * collect pass and the wave-per-ray compositing beside the per-lane one; both
+ * are joined back into `stream` with events before the call returns.
  * Cost model: the collect walk is bound by the dependent latency of its
  * node visits (about 95 4-wide node visits and 44 leaf tests per ray on a
  * 1M-Gaussian surface scene); compositing reads each list once. */
