@@ -150,6 +150,23 @@ struct TraceArgs {
     float *det_rec;
     uint32_t *det_keys;
     const uint32_t *det_off;
+    // stored-list backward: per-Gaussian rows of raw geometric sums (16
+    // floats: ga*g, pruning weight, v 3, S 6, pad) that the tiles' flushes add
+    // to, and a byte per Gaussian marking a touched row (k_raw_apply turns the
+    // rows into gradients and clears both)
+    float *raw;
+    uint8_t *raw_touched;
+    // per launch tile: set by the stored-list backward when the tile held more
+    // distinct Gaussians than its table; the recompute kernel then redoes the
+    // tile's stored lists hit by hit and clears the mark
+    uint8_t *bwd_overflow;
+    // stored-list backward: each tile's per-Gaussian sums as 64-byte records
+    // (colour 3, ga*g, pruning weight, v 3, S 6, view index), indexed tile *
+    // 128 + slot; per Gaussian the count of its records and the first
+    // kRecList record indices (later ones go to the raw row by atomics)
+    float *bwd_rec;
+    uint32_t *bwd_cnt;
+    uint32_t *bwd_lst;
 };
 
 // ---------------------------------------------------------------------------

@@ -62,6 +62,21 @@ struct gray_ctx {
     int64_t det_px_cap = 0;
     uint32_t *det_hist = nullptr, *det_sums = nullptr;
     int64_t det_hist_cap = 0, det_sums_cap = 0;
+    // stored-list backward: raw geometric gradient sums per Gaussian (kept
+    // zero between backward calls) and their touched marks
+    float *raw = nullptr;            // [raw_cap][16]
+    uint8_t *raw_touched = nullptr;  // [raw_cap]
+    int64_t raw_cap = 0;
+    uint8_t *bwd_overflow = nullptr; // [bwd_overflow_cap] per launch tile, zero between backward calls
+    int64_t bwd_overflow_cap = 0;
+    // stored-list backward: tile costs, the tiles in descending cost, the fetch counter
+    uint8_t *tile_cost = nullptr;    // [bwd_overflow_cap]
+    uint32_t *tile_order = nullptr;  // [bwd_overflow_cap]
+    int32_t *tile_next = nullptr;    // [1]
+    int stored_grid = 0;             // resident workgroups of the stored-list backward (0 = not queried)
+    float *bwd_rec = nullptr;        // [bwd_overflow_cap * 128][16] per-tile per-Gaussian sums
+    uint32_t *bwd_cnt = nullptr;     // [raw_cap] records per Gaussian, zero between backward calls
+    uint32_t *bwd_lst = nullptr;     // [raw_cap][4] record indices per Gaussian
     // timing
     bool timing = false;
     float last_build_ms = -1.0f;

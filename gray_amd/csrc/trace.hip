@@ -65,6 +65,17 @@ __device__ __forceinline__ bool slab_test(const LaneRay &r, const float *b, floa
 
 // XCD-aware block -> tile mapping: workgroup b runs on XCD (b % 8); give each
 // XCD a contiguous band of tiles so neighbouring tiles share that XCD's L2.
+__device__ __forceinline__ void tile_coords(const TraceArgs &A, int tile, int &view, int &tx, int &ty) {
+    int v = 0;
+#pragma unroll
+    for (int k = 1; k < GRAY_MAX_VIEWS; ++k)
+        if (k < A.num_views && tile >= A.views[k].tile_base) v = k;
+    view = v;
+    const int lt = tile - A.views[v].tile_base;
+    tx = lt % A.views[v].tiles_x;
+    ty = lt / A.views[v].tiles_x;
+}
+
 __device__ __forceinline__ bool block_tile(const TraceArgs &A, int &view, int &tx, int &ty, int *tile_out = nullptr) {
     const int b = blockIdx.x;
     const int tile = (b & 7) * A.xcd_chunk + (b >> 3);
@@ -1721,45 +1732,293 @@ __device__ __forceinline__ void det_put(const TraceArgs &A, size_t idx, uint32_t
     A.det_keys[idx] = gid;
 }
 
-// Per-wave gradient accumulator in LDS: the rays of a tile composite largely
-// the same Gaussians, so their terms are summed here (ds_add_f32) and each
-// Gaussian reaches the global tensors once per tile.  Open addressing with a
-// bounded probe; a Gaussian that finds no slot goes to the global tensors
-// directly.  Terms are stored [term][slot].
+__device__ __forceinline__ void geometry_terms(const TraceArgs &A, uint32_t gid, float clip, float eps_sg,
+                                               const float gl[3][3], float gxw0, float gxw1, float gxw2, float *t);
+
+// Raw per-hit sums of the stored-list backward.  Every term of a hit's
+// geometric gradient is linear in two per-hit quantities scaled by per-Gaussian
+// constants: with c = ga * dg/d(q) (q = |l|^2, l the local hit point times
+// clip) and K = MAX_ALPHA * sigmoid(opacity), the world hit gradient is
+// gxw = -K clip M^T (c l) and the L2W-row gradient gl = K clip M^T (c l l^T)
+// (M = W2L).  The table therefore sums per Gaussian: colour g w (3), ga g (1),
+// w (1), v = c l (3) and the symmetric S = c l l^T (xx xy xz yy yz zz, 6), and
+// the per-Gaussian transform to rotation, scale, mean and opacity gradients
+// runs once per Gaussian and tile at the flush instead of once per hit.
+constexpr int kRawTerms = 14;
+
+// Per-wave accumulator in LDS: the rays of a tile composite largely the same
+// Gaussians (about 42 distinct ones for 367 hits in an 8x8 tile of a 1M
+// frame), so their raw terms are summed here (ds_add_f32) and each Gaussian
+// reaches the global tensors once per tile.  Open addressing with a bounded
+// probe; a Gaussian that finds no slot is transformed and written at once.
+// Terms are stored [term][slot].
 constexpr int kGradSlots = 128;
 
-struct GradTable {
-    uint32_t *key;  // [kGradSlots], GRAY_EMPTY = free
-    float *val;     // [kGradTerms][kGradSlots]
+// Rotation, scale, mean, opacity and pruning terms (t[3..14] of the
+// kGradTerms order) of one Gaussian from its raw geometric sums r (the row
+// layout of TraceArgs::raw: ga*g, pruning weight, v 3, S xx xy xz yy yz zz).
+__device__ __forceinline__ void raw_geometry_terms(const TraceArgs &A, uint32_t gid, float eps_sg, const float *r,
+                                                   float *t) {
+#pragma clang fp contract(off)
+    const f4v *g4 = reinterpret_cast<const f4v *>(A.geo + gid);
+    const f4v ga = g4[0], gb = g4[1], gc = g4[2], gd = g4[3];
+    const float m[9] = {ga[0], ga[1], ga[2], ga[3], gb[0], gb[1], gb[2], gb[3], gc[0]};
+    const float clip = gd[1];
+    const float op = gray_sigmoid(A.opacity[gid]);
+    const float kc = (GRAY_MAX_ALPHA * op) * clip;
+    // gl[i][j] = K clip sum_a M[a][i] S[a][j]
+    const float S[3][3] = {{r[5], r[6], r[7]}, {r[6], r[8], r[9]}, {r[7], r[9], r[10]}};
+    float gl[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) gl[i][j] = kc * ((m[i] * S[0][j] + m[3 + i] * S[1][j]) + m[6 + i] * S[2][j]);
+    const float gxw0 = -kc * ((m[0] * r[2] + m[3] * r[3]) + m[6] * r[4]);
+    const float gxw1 = -kc * ((m[1] * r[2] + m[4] * r[3]) + m[7] * r[4]);
+    const float gxw2 = -kc * ((m[2] * r[2] + m[5] * r[3]) + m[8] * r[4]);
+    geometry_terms(A, gid, clip, eps_sg, gl, gxw0, gxw1, gxw2, t);
+    t[13] = ((GRAY_MAX_ALPHA * r[0]) * op) * (1.0f - op);
+    t[14] = r[1];
+}
+
+// The raw sums s (kRawTerms order) of one Gaussian to the device: the colour
+// sums to the view's colour gradient, the geometric sums to the Gaussian's raw
+// row (one 64-byte line), the pruning weight normalised by the view's pixels.
+__device__ __forceinline__ void raw_to_global(const TraceArgs &A, const DevView &V, uint32_t gid, const float *s,
+                                              float pixels) {
+    atomicAdd(&V.grad_rgb[3 * gid + 0], s[0]);
+    atomicAdd(&V.grad_rgb[3 * gid + 1], s[1]);
+    atomicAdd(&V.grad_rgb[3 * gid + 2], s[2]);
+    float *row = A.raw + 16 * (size_t)gid;
+    atomicAdd(&row[0], s[3]);
+    atomicAdd(&row[1], s[4] / pixels);
+#pragma unroll
+    for (int j = 5; j < kRawTerms; ++j) atomicAdd(&row[j - 3], s[j]);
+    A.raw_touched[gid] = 1;
+}
+
+// floats per slot row: 14 terms + 6 pad.  A stride of 80 B (20 banks) starts
+// the rows of 16 consecutive slots on 16 different 4-bank groups, so the
+// 16-byte accesses of lanes holding different slots spread over all banks.
+constexpr int kRawRow = 20;
+constexpr int kClaimProbes = 8;  // slots tried by a claim
+constexpr int kRecList = 4;      // records listed per Gaussian (stored-list backward)
+
+// The table of one wave.  It has no path to global memory inside the hit
+// loop: a store or atomic there (even on a branch never taken) makes the
+// compiler wait for every load in flight at each step, which would undo the
+// loop's prefetch.  A Gaussian that finds no slot (the tile holds about as
+// many distinct Gaussians as the table has slots) fails the add; the hits
+// concerned are added one by one after the loop.
+struct RawTable {
+    uint32_t *key;   // [kGradSlots], GRAY_EMPTY = free
+    float *val;      // [kGradSlots][kRawRow]
+    uint16_t *list;  // [kGradSlots] occupied slots, compacted at the flush
+    uint32_t *owner; // [kGradSlots] the lane that updates a slot in a step
 
     __device__ __forceinline__ void clear(int lane) {
         for (int i = lane; i < kGradSlots; i += GRAY_WAVE) key[i] = GRAY_EMPTY;
-        for (int i = lane; i < kGradSlots * kGradTerms; i += GRAY_WAVE) val[i] = 0.0f;
+        for (int i = lane; i < kGradSlots * kRawRow; i += GRAY_WAVE) val[i] = 0.0f;
     }
-    __device__ __forceinline__ void add(const TraceArgs &A, const DevView &V, uint32_t gid, const float *t) {
+    // The slot of gid (claimed if new), or -1 when a bounded probe finds none
+    // (the table is nearly full: a full table would cost every later new
+    // Gaussian a scan of all slots).
+    __device__ __forceinline__ int claim(uint32_t gid) const {
         uint32_t slot = (gid * 2654435761u) >> 25;  // 7 bits
-        for (int probe = 0; probe < 8; ++probe) {
-            const uint32_t k = atomicCAS(&key[slot], GRAY_EMPTY, gid);
-            if (k == GRAY_EMPTY || k == gid) {
-#pragma unroll
-                for (int j = 0; j < kGradTerms; ++j) atomicAdd(&val[j * kGradSlots + slot], t[j]);
-                return;
-            }
+        for (int probe = 0; probe < kClaimProbes; ++probe) {
+            // most hits find their Gaussian already in the table: a plain
+            // read first, the compare-and-swap only for a free slot
+            // (a plain read: a volatile one would lose the LDS address space
+            // and become a flat access that waits for every load in flight)
+            uint32_t k = key[slot];
+            if (k == gid) return (int)slot;
+            if (k == GRAY_EMPTY) k = atomicCAS(&key[slot], GRAY_EMPTY, gid);
+            if (k == GRAY_EMPTY || k == gid) return (int)slot;
             slot = (slot + 1) & (kGradSlots - 1);
         }
-        grad_to_global(A, V, gid, t);
+        return -1;
     }
-    __device__ __forceinline__ void flush(const TraceArgs &A, const DevView &V, int lane) const {
-        for (int i = lane; i < kGradSlots; i += GRAY_WAVE) {
-            const uint32_t gid = key[i];
-            if (gid == GRAY_EMPTY) continue;
-            float t[kGradTerms];
+    // The slot holding gid, or -1 (read-only: every key is in place).
+    __device__ __forceinline__ int lookup(uint32_t gid) const {
+        uint32_t slot = (gid * 2654435761u) >> 25;
+        for (int probe = 0; probe < kClaimProbes; ++probe) {
+            const uint32_t k = key[slot];
+            if (k == gid) return (int)slot;
+            if (k == GRAY_EMPTY) return -1;
+            slot = (slot + 1) & (kGradSlots - 1);
+        }
+        return -1;
+    }
+    // Called by the whole wave (uniform control flow); lanes with slot < 0
+    // take no part.  One lane per slot and step updates the slot's row by
+    // plain 16-byte reads and writes; lanes that share a slot with another
+    // lane (the same Gaussian away from the neighbours combined before) take
+    // later rounds.  The workgroup is one wave, so its LDS operations execute
+    // in order and the barriers below cost no wait on other waves.
+    __device__ __forceinline__ void add_at(int slot, const float *s, int lane) {
+        bool pend = slot >= 0;
+        while (__ballot(pend)) {
+            if (pend) owner[slot] = (uint32_t)lane;
+            __syncthreads();
+            const bool own = pend && owner[slot] == (uint32_t)lane;
+            if (own) {
+                f4v *row = reinterpret_cast<f4v *>(val + slot * kRawRow);
+                const f4v a = row[0], b = row[1], c = row[2], d = row[3];
+                row[0] = f4v{a[0] + s[0], a[1] + s[1], a[2] + s[2], a[3] + s[3]};
+                row[1] = f4v{b[0] + s[4], b[1] + s[5], b[2] + s[6], b[3] + s[7]};
+                row[2] = f4v{c[0] + s[8], c[1] + s[9], c[2] + s[10], c[3] + s[11]};
+                row[3] = f4v{d[0] + s[12], d[1] + s[13], d[2], d[3]};
+            }
+            __syncthreads();
+            pend = pend && !own;
+        }
+    }
+    // After a barrier: the occupied slots compacted into list; returns their count.
+    __device__ __forceinline__ int compact(int lane) {
+        static_assert(kGradSlots == 2 * GRAY_WAVE, "two slots per lane");
+        const bool o0 = key[lane] != GRAY_EMPTY, o1 = key[lane + GRAY_WAVE] != GRAY_EMPTY;
+        const uint64_t b0 = __ballot(o0), b1 = __ballot(o1);
+        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        const int n0 = __popcll(b0);
+        if (o0) list[__popcll(b0 & below)] = (uint16_t)lane;
+        if (o1) list[n0 + __popcll(b1 & below)] = (uint16_t)(lane + GRAY_WAVE);
+        __syncthreads();
+        return n0 + __popcll(b1);
+    }
+    // After compact(): every occupied slot's Gaussian, one per lane.  Its
+    // sums become record tile * kGradSlots + i, listed under the Gaussian
+    // (one integer atomic instead of a float atomic per sum); a Gaussian with
+    // kRecList records already adds to its raw row.
+    __device__ __forceinline__ void flush(const TraceArgs &A, const DevView &V, int lane, float pixels, int total,
+                                          int tile, int view) {
+        for (int i = lane; i < total; i += GRAY_WAVE) {
+            const int slot = list[i];
+            const uint32_t gid = key[slot];
+            float s[kRawTerms];
 #pragma unroll
-            for (int j = 0; j < kGradTerms; ++j) t[j] = val[j * kGradSlots + i];
-            grad_to_global(A, V, gid, t);
+            for (int j = 0; j < kRawTerms; ++j) s[j] = val[slot * kRawRow + j];
+            const uint32_t c = atomicAdd(&A.bwd_cnt[gid], 1u);
+            if (c < (uint32_t)kRecList) {
+                const uint32_t ri = (uint32_t)tile * kGradSlots + (uint32_t)i;
+                f4v *o = reinterpret_cast<f4v *>(A.bwd_rec + 16 * (size_t)ri);
+                o[0] = f4v{s[0], s[1], s[2], s[3]};
+                o[1] = f4v{s[4] / pixels, s[5], s[6], s[7]};
+                o[2] = f4v{s[8], s[9], s[10], s[11]};
+                o[3] = f4v{s[12], s[13], __int_as_float(view), 0.0f};
+                A.bwd_lst[(size_t)gid * kRecList + c] = ri;
+            } else {
+                raw_to_global(A, V, gid, s, pixels);
+            }
         }
     }
 };
+
+// Raw terms (kRawTerms order) of one composited hit of the stored-list
+// backward: the per-hit half of the reference backward (backward_pass.cu:
+// 32-95): the hit test and alpha from the snapshot record g (three 16-byte
+// words: W2L and mean, then the opacity and clip pair), the colour c, then the alpha gradient and
+// the running gradient gT of the transmittance behind the hit.  Returns false
+// if the ray misses the record.  With rebuild set, Ti is rebuilt from Tafter as
+// in backward_hit.
+__device__ __forceinline__ bool backward_hit_raw(const f4v &ga, const f4v &gb, const f4v &gc, const float2 &gd,
+                                                 float c0, float c1, float c2, const LaneRay &r, float p,
+                                                 const PixelState &ps, float Ti, float &gT, bool rebuild,
+                                                 float &Tafter, float *s) {
+#pragma clang fp contract(off)
+    const float m[9] = {ga[0], ga[1], ga[2], ga[3], gb[0], gb[1], gb[2], gb[3], gc[0]};
+    float dist, hx, hy, hz;
+    if (!gray_hit_test(m, gc[1], gc[2], gc[3], r.ox, r.oy, r.oz, r.dx, r.dy, r.dz, dist, hx, hy, hz)) return false;
+    const float op = gd.x, clip = gd.y;
+    float gv, q;
+    const float a = gray_alpha(hx, hy, hz, clip, op, p, gv, q);
+    if (rebuild) {
+        Ti = Tafter / (1.0f - a);
+        Tafter = Ti;
+    }
+    const float w = Ti * a;
+    s[0] = ps.g0 * w;
+    s[1] = ps.g1 * w;
+    s[2] = ps.g2 * w;
+    const float gdot = (ps.g0 * c0 + ps.g1 * c1) + ps.g2 * c2;
+    const float ioma = 1.0f / (1.0f - a);
+    float gal = (gdot - gT) * Ti;
+    gal += ((-ps.F) * ioma) * ps.bgdot;
+    gal += -((ps.Tf - ps.F) * ioma) * ps.estdot;
+    gT += (gdot - gT) * a;
+    s[3] = gal * gv;
+    s[4] = w;
+    float gsq;
+    if (p == 1.0f)
+        gsq = gv;
+    else if (p == 2.0f)
+        gsq = gv * q;
+    else if (p == 3.0f)
+        gsq = (gv * q) * q;
+    else
+        gsq = gv * powf(q, p - 1.0f);
+    const float c = gsq * gal;
+    const float lx = hx * clip, ly = hy * clip, lz = hz * clip;
+    const float cx = c * lx, cy = c * ly, cz = c * lz;
+    s[5] = cx;
+    s[6] = cy;
+    s[7] = cz;
+    s[8] = cx * lx;
+    s[9] = cx * ly;
+    s[10] = cx * lz;
+    s[11] = cy * ly;
+    s[12] = cy * lz;
+    s[13] = cz * lz;
+    return true;
+}
+
+// Rotation, scale and mean terms (t[3..12]) of one Gaussian from the gradient
+// gl of its L2W rows and the world-space hit gradient gxw, with the live scale
+// and rotation (reference backward_pass.cu:96-126, backward_normalize_act of
+// helpers.cu:9-34).  Linear in gl and gxw, so it serves one hit or the sums of
+// many.
+__device__ __forceinline__ void geometry_terms(const TraceArgs &A, uint32_t gid, float clip, float eps_sg,
+                                               const float gl[3][3], float gxw0, float gxw1, float gxw2, float *t) {
+#pragma clang fp contract(off)
+    // scale
+    const float s0 = expf(A.scale[3 * gid + 0]), s1 = expf(A.scale[3 * gid + 1]), s2 = expf(A.scale[3 * gid + 2]);
+    const float *L = A.l2w + 12 * (size_t)gid;
+    // the rotation columns L[:, j] / d_j: one reciprocal per column
+    const float i0 = 1.0f / (s0 * clip + eps_sg), i1 = 1.0f / (s1 * clip + eps_sg), i2 = 1.0f / (s2 * clip + eps_sg);
+    const float gs0 = ((gl[0][0] * (L[0] * i0) + gl[1][0] * (L[3] * i0)) + gl[2][0] * (L[6] * i0)) * s0;
+    const float gs1 = ((gl[0][1] * (L[1] * i1) + gl[1][1] * (L[4] * i1)) + gl[2][1] * (L[7] * i1)) * s1;
+    const float gs2 = ((gl[0][2] * (L[2] * i2) + gl[1][2] * (L[5] * i2)) + gl[2][2] * (L[8] * i2)) * s2;
+    // rotation matrix rows
+    const float g0x = gl[0][0] * s0, g0y = gl[0][1] * s1, g0z = gl[0][2] * s2;
+    const float g1x = gl[1][0] * s0, g1y = gl[1][1] * s1, g1z = gl[1][2] * s2;
+    const float g2x = gl[2][0] * s0, g2y = gl[2][1] * s1, g2z = gl[2][2] * s2;
+    const float qr = A.rotation[4 * gid + 0], qx = A.rotation[4 * gid + 1], qy = A.rotation[4 * gid + 2],
+                qz = A.rotation[4 * gid + 3];
+    const float qlen = sqrtf(((qr * qr + qx * qx) + qy * qy) + qz * qz);
+    const float iq = 1.0f / qlen;
+    const float nr = qr * iq, nx = qx * iq, ny = qy * iq, nz = qz * iq;
+    const float gr = (2.f * nx * (g2y - g1z) + 2.f * ny * (g0z - g2x)) + 2.f * nz * (g1x - g0y);
+    const float gx = ((-4.f * nx * (g1y + g2z) + 2.f * ny * (g0y + g1x)) + 2.f * nz * (g0z + g2x)) + 2.f * nr * (g2y - g1z);
+    const float gy = ((2.f * nx * (g0y + g1x) - 4.f * ny * (g0x + g2z)) + 2.f * nz * (g1z + g2y)) + 2.f * nr * (g0z - g2x);
+    const float gz = ((2.f * nx * (g0z + g2x) + 2.f * ny * (g1z + g2y)) - 4.f * nz * (g0x + g1y)) + 2.f * nr * (g1x - g0y);
+    // reference backward_normalize_act: (g - q (q.g) / |q|^2) / |q|
+    const float qdot = ((gr * qr + gx * qx) + gy * qy) + gz * qz;
+    const float iq3 = (iq * iq) * iq;
+    const float q0 = (qdot * -qr) * iq3 + gr * iq;
+    const float q1 = (qdot * -qx) * iq3 + gx * iq;
+    const float q2 = (qdot * -qy) * iq3 + gy * iq;
+    const float q3 = (qdot * -qz) * iq3 + gz * iq;
+
+    t[3] = q0;
+    t[4] = q1;
+    t[5] = q2;
+    t[6] = q3;
+    t[7] = gs0;
+    t[8] = gs1;
+    t[9] = gs2;
+    t[10] = -gxw0;
+    t[11] = -gxw1;
+    t[12] = -gxw2;
+}
 
 // Gradient of one composited hit; the terms go to sink(gid, terms).
 // Ti is the transmittance before the hit; with rebuild set it is instead
@@ -1822,45 +2081,7 @@ __device__ __forceinline__ void backward_hit(const TraceArgs &A, const DevView &
     const float gl[3][3] = {{-gxw0 * lx, -gxw0 * ly, -gxw0 * lz},
                             {-gxw1 * lx, -gxw1 * ly, -gxw1 * lz},
                             {-gxw2 * lx, -gxw2 * ly, -gxw2 * lz}};
-    // scale
-    const float s0 = expf(A.scale[3 * gid + 0]), s1 = expf(A.scale[3 * gid + 1]), s2 = expf(A.scale[3 * gid + 2]);
-    const float *L = A.l2w + 12 * (size_t)gid;
-    // the rotation columns L[:, j] / d_j: one reciprocal per column
-    const float i0 = 1.0f / (s0 * clip + eps_sg), i1 = 1.0f / (s1 * clip + eps_sg), i2 = 1.0f / (s2 * clip + eps_sg);
-    const float gs0 = ((gl[0][0] * (L[0] * i0) + gl[1][0] * (L[3] * i0)) + gl[2][0] * (L[6] * i0)) * s0;
-    const float gs1 = ((gl[0][1] * (L[1] * i1) + gl[1][1] * (L[4] * i1)) + gl[2][1] * (L[7] * i1)) * s1;
-    const float gs2 = ((gl[0][2] * (L[2] * i2) + gl[1][2] * (L[5] * i2)) + gl[2][2] * (L[8] * i2)) * s2;
-    // rotation matrix rows
-    const float g0x = gl[0][0] * s0, g0y = gl[0][1] * s1, g0z = gl[0][2] * s2;
-    const float g1x = gl[1][0] * s0, g1y = gl[1][1] * s1, g1z = gl[1][2] * s2;
-    const float g2x = gl[2][0] * s0, g2y = gl[2][1] * s1, g2z = gl[2][2] * s2;
-    const float qr = A.rotation[4 * gid + 0], qx = A.rotation[4 * gid + 1], qy = A.rotation[4 * gid + 2],
-                qz = A.rotation[4 * gid + 3];
-    const float qlen = sqrtf(((qr * qr + qx * qx) + qy * qy) + qz * qz);
-    const float iq = 1.0f / qlen;
-    const float nr = qr * iq, nx = qx * iq, ny = qy * iq, nz = qz * iq;
-    const float gr = (2.f * nx * (g2y - g1z) + 2.f * ny * (g0z - g2x)) + 2.f * nz * (g1x - g0y);
-    const float gx = ((-4.f * nx * (g1y + g2z) + 2.f * ny * (g0y + g1x)) + 2.f * nz * (g0z + g2x)) + 2.f * nr * (g2y - g1z);
-    const float gy = ((2.f * nx * (g0y + g1x) - 4.f * ny * (g0x + g2z)) + 2.f * nz * (g1z + g2y)) + 2.f * nr * (g0z - g2x);
-    const float gz = ((2.f * nx * (g0z + g2x) + 2.f * ny * (g1z + g2y)) - 4.f * nz * (g0x + g1y)) + 2.f * nr * (g1x - g0y);
-    // reference backward_normalize_act: (g - q (q.g) / |q|^2) / |q|
-    const float qdot = ((gr * qr + gx * qx) + gy * qy) + gz * qz;
-    const float iq3 = (iq * iq) * iq;
-    const float q0 = (qdot * -qr) * iq3 + gr * iq;
-    const float q1 = (qdot * -qx) * iq3 + gx * iq;
-    const float q2 = (qdot * -qy) * iq3 + gy * iq;
-    const float q3 = (qdot * -qz) * iq3 + gz * iq;
-
-    t[3] = q0;
-    t[4] = q1;
-    t[5] = q2;
-    t[6] = q3;
-    t[7] = gs0;
-    t[8] = gs1;
-    t[9] = gs2;
-    t[10] = -gxw0;
-    t[11] = -gxw1;
-    t[12] = -gxw2;
+    geometry_terms(A, gid, clip, eps_sg, gl, gxw0, gxw1, gxw2, t);
     t[13] = gop;
     t[14] = w / ps.pixels;
     sink(gid, t);
@@ -1893,16 +2114,34 @@ __device__ __forceinline__ bool load_pixel_state(const TraceArgs &A, const DevVi
 }
 
 // Rays whose composited list fits the store: replay it back to front.
-__global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored(TraceArgs A) {
-    __shared__ uint32_t s_key[kGradSlots];
-    __shared__ float s_val[kGradSlots * kGradTerms];
+// The rays of a tile composite largely the same Gaussians (an 8x8 tile of the
+// 1M-Gaussian training frame: about 367 hits on 42 distinct Gaussians).  The
+// wave first claims a table slot for each distinct Gaussian of its lists and
+// stages its snapshot record and colour in LDS once; the back-to-front loop
+// then reads records from LDS, and global memory only for its list entries
+// (coalesced, prefetched a step ahead), so the gathers per tile fall from one
+// per hit to one per Gaussian.  A Gaussian that found no slot is read from
+// global memory where it occurs and its hits are added after the loop.
+constexpr int kRecRow = 20;  // floats per staged record: W2L 9, mean 3, op, clip, colour 3, pad 3
+struct StoredLds {
+    uint32_t key[kGradSlots];
+    __attribute__((aligned(16))) float val[kGradSlots * kRawRow];
+    __attribute__((aligned(16))) float rec[kGradSlots * kRecRow];
+    uint16_t list[kGradSlots];
+    uint32_t owner[kGradSlots];
+};
+
+__device__ __forceinline__ void stored_tile(const TraceArgs &A, int tile, StoredLds &L) {
+    uint32_t *s_key = L.key;
+    float *s_val = L.val, *s_rec = L.rec;
+    uint16_t *s_list = L.list;
+    uint32_t *s_owner = L.owner;
     int vi, tx, ty;
-    if (!block_tile(A, vi, tx, ty)) return;
+    tile_coords(A, tile, vi, tx, ty);
     const DevView &V = A.views[vi];
     const int lane = threadIdx.x;
-    GradTable tab{s_key, s_val};
+    RawTable tab{s_key, s_val, s_list, s_owner};
     tab.clear(lane);
-    __syncthreads();
     const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
     const int pix = y * V.stride + x;
     const int n = (x < V.width && y < V.height) ? (int)V.n_comp[pix] : 0;
@@ -1916,60 +2155,303 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored(TraceArgs A
     int mmax = m;
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) mmax = max(mmax, __shfl_xor(mmax, off));
-    const float p = *A.knobs.exp_power, eps_sg = *A.knobs.eps_scale_grad;
-    const int R = V.ray_count;
-    float gT = 0.0f;
-    float Tafter = ps.Tf;  // fp16_hits: transmittances rebuilt from the final one
-    for (int it = 0; it < mmax; ++it) {
-        const int k = m - 1 - it;
-        float t[kGradTerms];
-#pragma unroll
-        for (int j = 0; j < kGradTerms; ++j) t[j] = 0.0f;
-        uint32_t gid = GRAY_EMPTY;
-        if (k >= 0) {
-            const uint32_t g = V.hit_ids[(size_t)k * R + pix];
-            const float Ti = V.hit_T[(size_t)k * R + pix];
-            auto keep = [&](uint32_t gg, const float *tt) {
-#pragma unroll
-                for (int j = 0; j < kGradTerms; ++j) t[j] = tt[j];
-                gid = gg;
-            };
-            backward_hit(A, V, r, p, eps_sg, ps, g, Ti, gT, keep, A.fp16 != 0, Tafter);
+    if (mmax == 0) return;
+    const float p = *A.knobs.exp_power;
+    const float pixels = (float)(V.width * V.height);
+    const size_t R = (size_t)V.ray_count;
+    const uint32_t *ids = V.hit_ids + pix;
+    const float *hT = V.hit_T + pix;
+    const float *rgb = V.rgb;
+    const bool rebuild = A.fp16 != 0;
+    __syncthreads();
+    // 1. a slot for every Gaussian of the tile's lists (entries read back to
+    // front, a step ahead; a Gaussian that finds none is looked up in vain
+    // below and read from global memory)
+    {
+        size_t off = (size_t)(m - 1) * R;
+        uint32_t g = m >= 1 ? ids[off] : GRAY_EMPTY;
+        for (int k = m - 1; k >= 0; --k) {
+            off -= R;
+            const uint32_t cur = g;
+            if (k >= 1) g = ids[off];
+            tab.claim(cur);
         }
-        // Neighbouring pixels often composite the same Gaussian at the same
-        // step: combining their terms first saves LDS table operations, which
-        // bound this kernel.  Horizontal neighbours (lane ^ 1) holding the
-        // same Gaussian: the even one adds both.
-        const uint32_t g1 = dpp_u32<0xB1, 0xF>(gid);
-        const bool pair1 = gid != GRAY_EMPTY && g1 == gid;
-#pragma unroll
-        for (int j = 0; j < kGradTerms; ++j) {
-            const float u = dpp_f32<0xB1, 0xF>(t[j]);
-            if (pair1) t[j] += u;
-        }
-        if (pair1 && (lane & 1)) gid = GRAY_EMPTY;
-        // then pixels two apart in the row (lane ^ 2): the left one adds both
-        const uint32_t g3 = dpp_u32<0x4E, 0xF>(gid);
-        const bool pair3 = gid != GRAY_EMPTY && g3 == gid;
-#pragma unroll
-        for (int j = 0; j < kGradTerms; ++j) {
-            const float u = dpp_f32<0x4E, 0xF>(t[j]);
-            if (pair3) t[j] += u;
-        }
-        if (pair3 && (lane & 2)) gid = GRAY_EMPTY;
-        // then vertical neighbours (lane ^ 8, same 16-lane row): the upper one adds both
-        const uint32_t g2 = dpp_u32<0x128, 0xF>(gid);
-        const bool pair2 = gid != GRAY_EMPTY && g2 == gid;
-#pragma unroll
-        for (int j = 0; j < kGradTerms; ++j) {
-            const float u = dpp_f32<0x128, 0xF>(t[j]);
-            if (pair2) t[j] += u;
-        }
-        if (pair2 && (lane & 8)) gid = GRAY_EMPTY;
-        if (gid != GRAY_EMPTY) tab.add(A, V, gid, t);
     }
     __syncthreads();
-    tab.flush(A, V, lane);
+    // 2. the record and colour of every claimed Gaussian, once
+    const int total = tab.compact(lane);
+    for (int i = lane; i < total; i += GRAY_WAVE) {
+        const int slot = s_list[i];
+        const uint32_t g = s_key[slot];
+        const f4v *g4 = reinterpret_cast<const f4v *>(A.geo + g);
+        const float2 oc = *reinterpret_cast<const float2 *>(g4 + 3);
+        f4v *o = reinterpret_cast<f4v *>(s_rec + slot * kRecRow);
+        o[0] = g4[0];
+        o[1] = g4[1];
+        o[2] = g4[2];
+        o[3] = f4v{oc.x, oc.y, rgb[3 * g + 0], rgb[3 * g + 1]};
+        o[4] = f4v{rgb[3 * g + 2], 0.0f, 0.0f, 0.0f};
+    }
+    __syncthreads();
+    // The kernel ends with its longest tiles, whose steps are sequential per
+    // ray; waves of long lists take precedence at issue, so they run near
+    // their own pace and the short ones fill the gaps.
+    if (mmax >= 16)
+        __builtin_amdgcn_s_setprio(3);
+    else if (mmax >= 8)
+        __builtin_amdgcn_s_setprio(2);
+    else if (mmax >= 4)
+        __builtin_amdgcn_s_setprio(1);
+    else
+        __builtin_amdgcn_s_setprio(0);
+    // 3. back to front
+    float gT = 0.0f;
+    float Tafter = ps.Tf;  // fp16_hits: transmittances rebuilt from the final one
+    // steps whose hit this lane adds itself after the loop (its Gaussian has
+    // no slot); a failure past step 63 drops the whole tile instead
+    uint64_t failed = 0;
+    bool far = false;
+    // list entries of hits k (id0, T0) and k-1 (id1, T1) are in registers or
+    // in flight; off2 is the offset of entry k-2, stepped down each step
+    uint32_t id0 = GRAY_EMPTY, id1 = GRAY_EMPTY;
+    float T0 = 0.0f, T1 = 0.0f;
+    if (m >= 1) {
+        id0 = ids[(size_t)(m - 1) * R];
+        T0 = hT[(size_t)(m - 1) * R];
+    }
+    if (m >= 2) {
+        id1 = ids[(size_t)(m - 2) * R];
+        T1 = hT[(size_t)(m - 2) * R];
+    }
+    size_t off2 = (size_t)(m - 3) * R;
+    for (int it = 0; it < mmax; ++it) {
+        const int k = m - 1 - it;
+        uint32_t id2 = GRAY_EMPTY;
+        float T2 = 0.0f;
+        float t[kRawTerms];  // read only where gid is set
+        uint32_t gid = GRAY_EMPTY;
+        int slot = -1;
+        if (k >= 0) {
+            slot = tab.lookup(id0);
+            f4v ra, rb, rc, rx;
+            float c2;
+            if (slot >= 0) {
+                // (all five words as 16-byte LDS loads: a scalar load here
+                // would be merged with the global one below into a flat load)
+                const f4v *q = reinterpret_cast<const f4v *>(s_rec + slot * kRecRow);
+                ra = q[0], rb = q[1], rc = q[2], rx = q[3];
+                const f4v q4 = q[4];
+                c2 = q4[0];
+            } else {
+                const f4v *g4 = reinterpret_cast<const f4v *>(A.geo + id0);
+                const float2 oc = *reinterpret_cast<const float2 *>(g4 + 3);
+                ra = g4[0], rb = g4[1], rc = g4[2];
+                rx = f4v{oc.x, oc.y, rgb[3 * id0 + 0], rgb[3 * id0 + 1]};
+                c2 = rgb[3 * id0 + 2];
+            }
+            if (backward_hit_raw(ra, rb, rc, float2{rx[0], rx[1]}, rx[2], rx[3], c2, r, p, ps, T0, gT, rebuild,
+                                 Tafter, t)) {
+                if (slot >= 0) {
+                    gid = id0;
+                } else if (it < 64) {
+                    failed |= 1ull << it;
+                } else {
+                    far = true;
+                }
+            }
+        }
+        // The list entry two steps on, in flight during this hit's table
+        // update and the whole next step.  Issued after the record is in
+        // registers: a load issued earlier would be waited for with the
+        // record loads of a Gaussian without a slot (one counter, in order).
+        if (k >= 2) {
+            id2 = ids[off2];
+            T2 = hT[off2];
+        }
+        off2 -= R;
+        // Neighbouring pixels often composite the same Gaussian at the same
+        // step: combining their terms first saves table updates.  Pairs are
+        // symmetric (both lanes hold the same Gaussian, so the same slot), so
+        // the adds run with only the paired lanes active and read their
+        // partner.  Horizontal neighbours (lane ^ 1): the even one keeps both.
+        {
+            const uint32_t g1 = dpp_u32<0xB1, 0xF>(gid);
+            if (gid != GRAY_EMPTY && g1 == gid) {
+#pragma unroll
+                for (int j = 0; j < kRawTerms; ++j) t[j] += dpp_f32<0xB1, 0xF>(t[j]);
+                if (lane & 1) gid = GRAY_EMPTY;
+            }
+        }
+        // then pixels two apart in the row (lane ^ 2): the left one keeps both
+        {
+            const uint32_t g3 = dpp_u32<0x4E, 0xF>(gid);
+            if (gid != GRAY_EMPTY && g3 == gid) {
+#pragma unroll
+                for (int j = 0; j < kRawTerms; ++j) t[j] += dpp_f32<0x4E, 0xF>(t[j]);
+                if (lane & 2) gid = GRAY_EMPTY;
+            }
+        }
+        // then vertical neighbours (lane ^ 8, same 16-lane row): the upper one keeps both
+        {
+            const uint32_t g2 = dpp_u32<0x128, 0xF>(gid);
+            if (gid != GRAY_EMPTY && g2 == gid) {
+#pragma unroll
+                for (int j = 0; j < kRawTerms; ++j) t[j] += dpp_f32<0x128, 0xF>(t[j]);
+                if (lane & 8) gid = GRAY_EMPTY;
+            }
+        }
+        tab.add_at(gid != GRAY_EMPTY ? slot : -1, t, lane);
+        id0 = id1;
+        T0 = T1;
+        id1 = id2;
+        T1 = T2;
+    }
+    if (__ballot(far)) {
+        // a failure past the step mask: the recompute kernel redoes the tile
+        if (lane == 0) A.bwd_overflow[tile] = 1;
+        return;
+    }
+    __syncthreads();
+    tab.flush(A, V, lane, pixels, total, tile, vi);
+    // hits whose Gaussian found no slot: replay the list to them, each
+    // straight to the device
+    if (failed) {
+        float gT2 = 0.0f, Tafter2 = ps.Tf;
+        for (int it = 0; it < m && it < 64 && (failed >> it) != 0; ++it) {
+            const int k = m - 1 - it;
+            const uint32_t g = ids[(size_t)k * R];
+            const f4v *g4 = reinterpret_cast<const f4v *>(A.geo + g);
+            float t[kRawTerms];
+            if (backward_hit_raw(g4[0], g4[1], g4[2], *reinterpret_cast<const float2 *>(g4 + 3), rgb[3 * g + 0],
+                                 rgb[3 * g + 1], rgb[3 * g + 2], r, p, ps, hT[(size_t)k * R], gT2, rebuild, Tafter2,
+                                 t) &&
+                ((failed >> it) & 1))
+                raw_to_global(A, V, g, t, pixels);
+        }
+    }
+}
+
+// The stored-list backward runs persistent: as many workgroups as fit on the
+// device take tiles from a list ordered by cost (the tile's longest stored
+// list, longest first), one atomic per tile.  A tile's steps are sequential,
+// so the longest tiles set the kernel's end: started first, they run beside
+// the short ones instead of after them, and no workgroup slot waits on the
+// dispatcher.
+__global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_stored(TraceArgs A, const uint32_t *__restrict__ order,
+                                                                    int32_t *__restrict__ next) {
+    __shared__ StoredLds L;
+    __shared__ int s_t;
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(next, 1);
+        __syncthreads();
+        const int t = s_t;
+        __syncthreads();
+        if (t >= A.total_tiles) break;
+        stored_tile(A, (int)order[t], L);
+        __syncthreads();
+    }
+}
+
+// Cost of each tile for the stored-list backward: the longest list of its rays
+// that the store holds (0: nothing to do), clamped to 255.
+__global__ __launch_bounds__(GRAY_WAVE) void k_tile_cost(TraceArgs A, uint8_t *__restrict__ cost) {
+    int vi, tx, ty, tile;
+    if (!block_tile(A, vi, tx, ty, &tile)) return;
+    const DevView &V = A.views[vi];
+    const int lane = threadIdx.x;
+    const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
+    int n = (x < V.width && y < V.height) ? (int)V.n_comp[y * V.stride + x] : 0;
+    n = n <= A.capacity ? n : 0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) n = max(n, __shfl_xor(n, off));
+    if (lane == 0) cost[tile] = (uint8_t)min(n, 255);
+}
+
+// Tiles ordered by descending cost (a counting sort in one workgroup), the
+// fetch counter of k_trace_backward_stored reset.
+constexpr int kOrderThreads = 1024;
+__global__ __launch_bounds__(kOrderThreads) void k_tile_order(const uint8_t *__restrict__ cost, int tiles,
+                                                            uint32_t *__restrict__ order, int32_t *__restrict__ next) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t base[256];
+    const int t = threadIdx.x;
+    if (t < 256) hist[t] = 0;
+    __syncthreads();
+    for (int i = t; i < tiles; i += kOrderThreads) atomicAdd(&hist[255 - cost[i]], 1u);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t acc = 0;
+        for (int b = 0; b < 256; ++b) {
+            base[b] = acc;
+            acc += hist[b];
+        }
+        *next = 0;
+    }
+    __syncthreads();
+    for (int i = t; i < tiles; i += kOrderThreads) order[atomicAdd(&base[255 - cost[i]], 1u)] = (uint32_t)i;
+}
+
+// Gradients of the stored-list backward from the per-Gaussian raw rows, one
+// thread per Gaussian: a touched row becomes the rotation, scale, mean,
+// opacity and pruning terms (raw_geometry_terms, with the live parameters),
+// added to the reference gradient tensors; the row and its mark are cleared
+// for the next backward.  No other thread writes these elements, so the adds
+// are plain read-modify-writes.
+__global__ __launch_bounds__(256) void k_raw_apply(TraceArgs A, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = min(A.bwd_cnt[i], (uint32_t)kRecList);
+    const bool touched = A.raw_touched[i] != 0;
+    if (c == 0 && !touched) return;
+    float r[11] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    f4v *row = reinterpret_cast<f4v *>(A.raw + 16 * (size_t)i);
+    if (touched) {
+        const f4v r0 = row[0], r1 = row[1], r2 = row[2];
+        r[0] = r0[0], r[1] = r0[1], r[2] = r0[2], r[3] = r0[3], r[4] = r1[0], r[5] = r1[1];
+        r[6] = r1[2], r[7] = r1[3], r[8] = r2[0], r[9] = r2[1], r[10] = r2[2];
+    }
+    // the tiles' records: colour to the view's colour gradient, the rest summed
+    for (uint32_t j = 0; j < c; ++j) {
+        const f4v *q = reinterpret_cast<const f4v *>(A.bwd_rec + 16 * (size_t)A.bwd_lst[(size_t)i * kRecList + j]);
+        const f4v q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        float *grgb = A.views[__float_as_int(q3[2])].grad_rgb + 3 * (size_t)i;
+        grgb[0] += q0[0];
+        grgb[1] += q0[1];
+        grgb[2] += q0[2];
+        r[0] += q0[3];
+        r[1] += q1[0];
+        r[2] += q1[1], r[3] += q1[2], r[4] += q1[3];
+        r[5] += q2[0], r[6] += q2[1], r[7] += q2[2], r[8] += q2[3];
+        r[9] += q3[0], r[10] += q3[1];
+    }
+    float t[kGradTerms];
+    raw_geometry_terms(A, (uint32_t)i, *A.knobs.eps_scale_grad, r, t);
+    A.g_rot[4 * i + 0] += t[3];
+    A.g_rot[4 * i + 1] += t[4];
+    A.g_rot[4 * i + 2] += t[5];
+    A.g_rot[4 * i + 3] += t[6];
+    A.g_scale[3 * i + 0] += t[7];
+    A.g_scale[3 * i + 1] += t[8];
+    A.g_scale[3 * i + 2] += t[9];
+    A.g_mean[3 * i + 0] += t[10];
+    A.g_mean[3 * i + 1] += t[11];
+    A.g_mean[3 * i + 2] += t[12];
+    A.g_op[i] += t[13];
+    if (A.g_prune) A.g_prune[i] += t[14];
+    if (A.visible) A.visible[i] = 1;
+    A.bwd_cnt[i] = 0;
+    if (touched) {
+        const f4v z = {0.0f, 0.0f, 0.0f, 0.0f};
+        row[0] = z;
+        row[1] = z;
+        row[2] = z;
+        A.raw_touched[i] = 0;
+    }
+}
+
+__global__ void k_zero_u32(uint32_t *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
 }
 
 // Deterministic mode, rays whose composited list fits the store: each hit's
@@ -2012,8 +2494,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
     __shared__ uint32_t s_ki[GRAY_KB * GRAY_WAVE];
     __shared__ float s_ka[GRAY_KB * GRAY_WAVE];
     __shared__ uint32_t s_stack[GRAY_STACK * GRAY_WAVE];
-    int vi, tx, ty;
-    if (!block_tile(A, vi, tx, ty)) return;
+    int vi, tx, ty, tile;
+    if (!block_tile(A, vi, tx, ty, &tile)) return;
     const DevView &V = A.views[vi];
     const int lane = threadIdx.x;
     const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
@@ -2021,6 +2503,25 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_trace_backward_recompute(TraceArg
     const int pix = y * V.stride + x;
     int n = 0;
     if (inside) n = (int)V.n_comp[pix];
+    if (!kDet && A.bwd_overflow && A.bwd_overflow[tile]) {
+        // the stored-list backward dropped this tile (its table was full):
+        // its stored lists, hit by hit, straight to the gradient tensors
+        if (n > 0 && n <= A.capacity) {
+            LaneRay r0;
+            PixelState ps0;
+            if (load_pixel_state(A, V, pix, r0, ps0)) {
+                const float p0 = *A.knobs.exp_power, eps0 = *A.knobs.eps_scale_grad;
+                const size_t R = (size_t)V.ray_count;
+                float gT = 0.0f, Tafter = ps0.Tf;
+                for (int k = n - 1; k >= 0; --k)
+                    backward_hit(A, V, r0, p0, eps0, ps0, V.hit_ids[(size_t)k * R + pix], V.hit_T[(size_t)k * R + pix],
+                                 gT, [&](uint32_t g, const float *t) { grad_to_global(A, V, g, t); }, A.fp16 != 0,
+                                 Tafter);
+            }
+        }
+        __syncthreads();
+        if (lane == 0) A.bwd_overflow[tile] = 0;
+    }
     bool need = inside && n > A.capacity;
     if (!__ballot(need)) return;
     LaneRay r{};
@@ -2766,12 +3267,86 @@ extern "C" int gray_render_backward(gray_ctx *ctx, const gray_render_args *args,
     }
     if (n_scene == 0) return GRAY_OK;
     hipStream_t s = (hipStream_t)stream;
+    const bool stored = !args->deterministic && args->hit_capacity > 0;
+    if (stored && ctx->raw_cap < n_scene) {
+        // rows of raw sums, zero between backward calls (k_raw_apply clears what it reads)
+        GRAY_HIP(hipFree(ctx->raw));
+        GRAY_HIP(hipFree(ctx->raw_touched));
+        GRAY_HIP(hipFree(ctx->bwd_cnt));
+        GRAY_HIP(hipFree(ctx->bwd_lst));
+        ctx->raw = nullptr;
+        ctx->raw_touched = nullptr;
+        ctx->bwd_cnt = nullptr;
+        ctx->bwd_lst = nullptr;
+        ctx->raw_cap = 0;
+        const int64_t words = (n_scene + 3) / 4;
+        if (hipMalloc(&ctx->raw, sizeof(float) * 16 * n_scene) != hipSuccess ||
+            hipMalloc(&ctx->raw_touched, sizeof(uint32_t) * words) != hipSuccess ||
+            hipMalloc(&ctx->bwd_cnt, sizeof(uint32_t) * n_scene) != hipSuccess ||
+            hipMalloc(&ctx->bwd_lst, sizeof(uint32_t) * kRecList * n_scene) != hipSuccess) {
+            (void)hipGetLastError();
+            gray_set_error(GRAY_ERR_NOMEM, "render_backward: cannot allocate the raw gradient rows (%lld bytes)",
+                           (long long)(64 * n_scene));
+            return GRAY_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL(k_zero_u32, dim3(2048), dim3(256), 0, s, reinterpret_cast<uint32_t *>(ctx->raw),
+                           16 * n_scene);
+        hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, s, reinterpret_cast<uint32_t *>(ctx->raw_touched),
+                           words);
+        hipLaunchKernelGGL(k_zero_u32, dim3(256), dim3(256), 0, s, ctx->bwd_cnt, n_scene);
+        GRAY_HIP(hipGetLastError());
+        ctx->raw_cap = n_scene;
+    }
+    if (stored) {
+        // per-tile overflow marks of the launch with the most tiles
+        int64_t tiles = 0;
+        for (int first = 0; first < args->num_views; first += GRAY_MAX_VIEWS) {
+            TraceArgs T;
+            if ((rc = fill_args(ctx, args, first, std::min(GRAY_MAX_VIEWS, args->num_views - first), T))) return rc;
+            tiles = std::max<int64_t>(tiles, T.total_tiles);
+        }
+        if (ctx->bwd_overflow_cap < tiles) {
+            GRAY_HIP(hipFree(ctx->bwd_overflow));
+            GRAY_HIP(hipFree(ctx->tile_cost));
+            GRAY_HIP(hipFree(ctx->tile_order));
+            GRAY_HIP(hipFree(ctx->tile_next));
+            GRAY_HIP(hipFree(ctx->bwd_rec));
+            ctx->bwd_rec = nullptr;
+            ctx->bwd_overflow = nullptr;
+            ctx->tile_cost = nullptr;
+            ctx->tile_order = nullptr;
+            ctx->tile_next = nullptr;
+            ctx->bwd_overflow_cap = 0;
+            const int64_t words = (tiles + 3) / 4;
+            if (hipMalloc(&ctx->bwd_overflow, sizeof(uint32_t) * words) != hipSuccess ||
+                hipMalloc(&ctx->tile_cost, sizeof(uint32_t) * words) != hipSuccess ||
+                hipMalloc(&ctx->tile_order, sizeof(uint32_t) * tiles) != hipSuccess ||
+                hipMalloc(&ctx->tile_next, sizeof(int32_t)) != hipSuccess ||
+                hipMalloc(&ctx->bwd_rec, sizeof(float) * 16 * kGradSlots * tiles) != hipSuccess) {
+                (void)hipGetLastError();
+                gray_set_error(GRAY_ERR_NOMEM, "render_backward: cannot allocate the tile overflow marks");
+                return GRAY_ERR_NOMEM;
+            }
+            hipLaunchKernelGGL(k_zero_u32, dim3(64), dim3(256), 0, s, reinterpret_cast<uint32_t *>(ctx->bwd_overflow),
+                               words);
+            GRAY_HIP(hipGetLastError());
+            ctx->bwd_overflow_cap = tiles;
+        }
+    }
+    bool apply = false;
+    TraceArgs last;
     // deterministic: one view per launch, so each Gaussian's sum runs in one thread
     const int chunk = args->deterministic ? 1 : GRAY_MAX_VIEWS;
     for (int first = 0; first < args->num_views; first += chunk) {
         const int nv = std::min(chunk, args->num_views - first);
         TraceArgs A;
         if ((rc = fill_args(ctx, args, first, nv, A))) return rc;
+        A.raw = ctx->raw;
+        A.raw_touched = ctx->raw_touched;
+        A.bwd_rec = ctx->bwd_rec;
+        A.bwd_cnt = ctx->bwd_cnt;
+        A.bwd_lst = ctx->bwd_lst;
+        A.bwd_overflow = stored ? ctx->bwd_overflow : nullptr;
         for (int i = 0; i < nv; ++i) {
             const DevView &d = A.views[i];
             if (!d.out_T || !d.out_F || !d.out_est || !d.out_ro || !d.out_rd || !d.n_comp || !d.comp_rgb ||
@@ -2796,12 +3371,30 @@ extern "C" int gray_render_backward(gray_ctx *ctx, const gray_render_args *args,
             continue;
         }
         if (A.capacity > 0) {
-            hipLaunchKernelGGL(k_trace_backward_stored, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A);
+            if (ctx->stored_grid == 0) {
+                int per_cu = 0, cus = 0;
+                GRAY_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace_backward_stored, GRAY_WAVE, 0));
+                GRAY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+                ctx->stored_grid = std::max(1, per_cu) * std::max(1, cus);
+            }
+            hipLaunchKernelGGL(k_tile_cost, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, ctx->tile_cost);
+            hipLaunchKernelGGL(k_tile_order, dim3(1), dim3(kOrderThreads), 0, s, ctx->tile_cost, A.total_tiles,
+                               ctx->tile_order, ctx->tile_next);
+            hipLaunchKernelGGL(k_trace_backward_stored, dim3(std::min(ctx->stored_grid, A.total_tiles)),
+                               dim3(GRAY_WAVE), 0, s, A, ctx->tile_order, ctx->tile_next);
             GRAY_HIP(hipGetLastError());
+            apply = true;
+            last = A;
         }
         hipLaunchKernelGGL(k_trace_backward_recompute<false>, dim3(8 * A.xcd_chunk), dim3(GRAY_WAVE), 0, s, A, A.geo,
                            A.nodes);
         GRAY_HIP(hipGetLastError());
+        if (apply) {
+            // this launch's records (their view indices are the launch's)
+            hipLaunchKernelGGL(k_raw_apply, dim3(gray_div_up(n_scene, 256)), dim3(256), 0, s, last, (int)n_scene);
+            GRAY_HIP(hipGetLastError());
+            apply = false;
+        }
     }
     return GRAY_OK;
 }

@@ -186,6 +186,91 @@ def test_backward_matches_oracle(tiny):
     assert np.array_equal(vis > 0, (np.abs(ob["opacity"].reshape(-1)) > 0) | (ob["pruning"].reshape(-1) > 0))
 
 
+def pixel_grid_scene(cam, blocks, logit=0.0):
+    """Tiny Gaussians on pixel rays: for each (depths, columns) block, one per
+    depth on the ray of every pixel in those columns.  Each is hit by its own
+    pixel's ray only (radius under a seventh of the pixel spacing at these
+    depths, centre a third of it off the ray), so an 8x8 tile of a block composites 64 x len(depths) distinct
+    Gaussians."""
+    w, h = cam.image_width, cam.image_height
+    ro, rd = O.pinhole_rays(w, h, cam.fov_y, cam.origin.astype(np.float32), cam.rotation_c2w_blender())
+    means = []
+    for depths, columns in blocks:
+        pix = np.array([y * w + x for y in range(h) for x in columns])
+        means += [ro[pix] + t * rd[pix] for t in depths]
+    mean = np.concatenate(means)
+    n = mean.shape[0]
+    gen = torch.Generator().manual_seed(11)
+    # off the ray by about a third of the radius, so the hit is not at the
+    # centre, where the rotation and scale gradients vanish to rounding
+    off = torch.randn(n, 3, generator=gen).numpy()
+    mean = (mean + 0.002 * off / np.linalg.norm(off, axis=1, keepdims=True)).astype(np.float32)
+    rotation = torch.nn.functional.normalize(torch.randn(n, 4, generator=gen), dim=1)
+    scale = torch.full((n, 3), -5.8) + 0.2 * torch.rand(n, 3, generator=gen)
+    opacity = logit + torch.randn(n, 1, generator=gen) * 0.5
+    sh_dc = 0.5 + 0.2 * torch.randn(n, 1, 3, generator=gen)
+    sh_rest = 0.05 * torch.randn(n, 15, 3, generator=gen)
+    return scenes.GaussianScene(torch.from_numpy(mean), rotation, scale, opacity, sh_dc, sh_rest)
+
+
+@pytest.mark.parametrize("size,blocks,logit,cap", [
+    (64, [((2.6, 3.0, 3.4), range(64))], 0.0, 16),
+    (64, [((2.6, 3.0, 3.4), range(32)), ((3.0,), range(32, 64))], 0.0, 16),
+    (16, [(tuple(np.linspace(2.0, 4.0, 70)), range(16))], -3.5, 80),
+], ids=["every-tile", "half", "deep"])
+def test_backward_table_overflow(size, blocks, logit, cap):
+    """Tiles whose composited hits span more distinct Gaussians than the
+    stored-list backward's per-tile table holds (128 slots): three Gaussians on
+    every pixel's ray make 192 per 8x8 tile: the hits whose Gaussian found no
+    slot (and those merged into them) are added hit by hit after the table's
+    flush.  In the "half" frame the right half has one Gaussian per pixel (64
+    per tile), so tables that overflow and tables that do not share the launch.
+    In the "deep" frame 70 faint Gaussians lie on every ray, so slots run out
+    past the 64 steps a lane can mark: those tiles drop their table and the
+    recompute kernel redoes their stored lists.  Gradients equal the oracle's
+    element by element, on two backward calls in a row (the first clears the
+    marks it leaves)."""
+    cam = scenes.tiny_camera(size, size)
+    scene = pixel_grid_scene(cam, blocks, logit)
+    nr = setup_native(scene, cam, t_threshold=0.0001, hit_capacity=cap, sh=False, channels=scene.channels())
+    torch.set_grad_enabled(True)
+    nr.forward_pass()
+    fb = nr.get_framebuffer()
+    g = nr.get_gaussians()
+    ncomp = nr.hits.n_comp[: size * size].cpu().numpy().reshape(size, size)
+    for depths, columns in blocks:
+        # every Gaussian of a short column is composited; a deep column passes step 64
+        assert (ncomp[:, list(columns)] >= min(len(depths), 65)).all(), "grid pixels composite their Gaussians"
+    gout = torch.randn(fb.grad_output_channels.shape, generator=torch.Generator().manual_seed(4)).cuda()
+    fb.grad_output_channels.copy_(gout)
+    names = ("grad_mean", "grad_rotation", "grad_scale", "grad_opacity", "grad_channels", "pruning_weight")
+    runs = []
+    for _ in range(2):
+        with torch.no_grad():
+            for name in names:
+                getattr(g, name).zero_()
+        nr.backward_pass()
+        torch.cuda.synchronize()
+        nr.device_status()
+        runs.append({k: getattr(g, k).cpu().numpy().copy() for k in names})
+    geo, _ = nr.debug_snapshot()
+    geo = geo.cpu().numpy()
+    _, l2w = O.snapshot(scene.mean, scene.rotation, scene.scale, scene.opacity, 0.01, 2.0)
+    ro, rd = rays_of(nr)
+    rgb = g.channels.cpu().numpy()
+    k = oracle_knobs(nr)
+    fwd = O.render(geo, rgb, ro, rd, k)
+    ob = O.backward(geo, l2w, rgb, scene.rotation, scene.scale, scene.opacity, ro, rd, fwd,
+                    gout[:size, :size].reshape(-1, 3).cpu().numpy(), k, size * size)
+    ok = comparable_gaussians(ncomp.reshape(-1), fwd, geo, ro, rd)
+    for run in runs:
+        assert_grads_close([
+            ("channels", run["grad_channels"], ob["rgb"]), ("opacity", run["grad_opacity"], ob["opacity"]),
+            ("scale", run["grad_scale"], ob["scale"]), ("rotation", run["grad_rotation"], ob["rotation"]),
+            ("mean", run["grad_mean"], ob["mean"]), ("pruning", run["pruning_weight"], ob["pruning"]),
+        ], ok)
+
+
 def test_recompute_backward_equals_stored():
     scene = scenes.tiny_scene(1024, seed=3)
     cam = scenes.tiny_camera(48, 40)
