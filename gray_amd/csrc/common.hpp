@@ -127,6 +127,7 @@ struct TraceArgs {
     // list_n[g] = hits, < 0 = incomplete (the ray is re-traversed)
     float4 *list_rec;
     int32_t *list_n;
+    uint8_t *tile_open;        // [tiles] 1: a ray of the tile handed work on as tasks (its lists are not final)
     int64_t list_stride;
     int32_t list_cap;
     int32_t budget;            // iterations of a collect walk before it hands its remaining work on

@@ -47,6 +47,7 @@ struct gray_ctx {
     // hit lists of the forward collect pass (one launch of <= 8 views)
     float4 *list_rec = nullptr;     // [list_entries] {dist, id bits, alpha, 0}
     int32_t *list_n = nullptr;
+    uint8_t *tile_open = nullptr;   // [list_lanes_cap / 64]: tile handed work on (k_collect)
     int64_t list_entries = 0;       // allocated records
     int64_t list_lanes_cap = 0;     // allocated entries of list_n
     int64_t list_lanes = 0;         // lanes of the last collecting launch

@@ -525,7 +525,7 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
                                                   const Bvh4Node *__restrict__ wide, const LaneRay &r, size_t g,
                                                   uint32_t start, int budget, uint32_t *__restrict__ s_stack,
                                                   uint32_t *__restrict__ s_leaf, int lane, int &cnt, bool &ok,
-                                                  int &visits, int &leaves) {
+                                                  bool &handed, int &visits, int &leaves) {
     constexpr int NL = kCollectLeaves;
     const int cap = A.list_cap;
     const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power;
@@ -563,6 +563,7 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
                 mmax = max(mmax, __shfl_xor(mmax, off));
                 tot += __shfl_xor(tot, off);
             }
+            handed = handed || m > 0;
             if (tot > 0) {
                 int base = 0;
                 if (lane == 0) base = atomicAdd(A.task_count_out, tot);
@@ -630,6 +631,7 @@ __device__ __forceinline__ void collect_walk(const TraceArgs &A, const GeoRecord
                     s_stack[sp * GRAY_WAVE + lane] = c;
                     ++sp;
                 } else if (budget >= 0) {
+                    handed = true;
                     const int o = atomicAdd(A.task_count_out, 1);
                     if (o < A.task_cap) A.task_out[o] = make_uint2((uint32_t)g, c);
                     else ok = false;
@@ -683,10 +685,12 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
     if (x < V.width && y < V.height) active = make_ray(A.knobs, V, x, y, r);
     const size_t g = (size_t)tile * GRAY_WAVE + lane;
     int cnt = 0, visits = 0, leaves = 0;
-    bool ok = true;
+    bool ok = true, handed = false;
     collect_walk<kStack>(A, geo, nodes, r, g, (active && A.num_gaussians > 0) ? 0u : GRAY_EMPTY,
-                                      A.budget, s_stack, s_leaf, lane, cnt, ok, visits, leaves);
+                                      A.budget, s_stack, s_leaf, lane, cnt, ok, handed, visits, leaves);
     A.list_n[g] = ok ? cnt : -1;
+    const bool open = __ballot(handed) != 0;
+    if (lane == 0 && A.tile_open) A.tile_open[tile] = open ? 1 : 0;
     if (A.tile_stats) {
         int hsum = ok ? cnt : 0;
 #pragma unroll
@@ -1281,9 +1285,14 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_heavy_queue(TraceArgs A) {
 // scan sums the unflagged entries, which are exactly the skip set (hits the
 // k-buffer never handed to compositing, hits beyond zfar, hits after
 // termination).
-__global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, const float *__restrict__ rgb0) {
+//
+// phase 0 takes every tile; phase 1 only the tiles whose lists the per-tile
+// walk finished (it runs beside the pooled task round), phase 2 the others.
+__global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, const float *__restrict__ rgb0,
+                                                               int phase) {
     int vi, tx, ty, tile;
     if (!block_tile(A, vi, tx, ty, &tile)) return;
+    if (phase != 0 && (A.tile_open[tile] != 0) != (phase == 2)) return;
     const DevView &V = A.views[vi];
     const int lane = threadIdx.x;
     const int x = tx * GRAY_TILE + (lane & 7), y = ty * GRAY_TILE + (lane >> 3);
@@ -1420,7 +1429,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
     // and padding entries get weight 0, which leaves the sums bit for bit as
     // if skipped) while the next batch's ids and alphas are in flight.
     float skc0 = 0.0f, skc1 = 0.0f, skc2 = 0.0f, ska = 0.0f, pnc = 1.0f;
-    const int nskip = traced ? n : 0;
+    // (empty when every entry was composited)
+    const int nskip = traced && (int)ncomp < n ? n : 0;
     float2 sx[kListBatch];
     auto fetch_skip = [&](int i0) {
 #pragma unroll
@@ -1439,7 +1449,9 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
             const uint64_t mw = i < 64 ? mask0 : mask1;
             const bool marked = i < kMaskEntries ? ((mw >> (i & 63)) & 1ull) != 0 : (id & kListMark) != 0;
             w[j] = marked ? 0.0f : sx[j].y;  // fp32 alpha; its fp16 rounding enters the sums below
-            const uint32_t gid = id & ~kListMark;
+            // entries of weight 0 read Gaussian 0's colour (a cached line)
+            // instead of gathering one they do not use
+            const uint32_t gid = w[j] != 0.0f ? (id & ~kListMark) : 0u;
             c[j][0] = rgb[3 * gid + 0];
             c[j][1] = rgb[3 * gid + 1];
             c[j][2] = rgb[3 * gid + 2];
@@ -1458,7 +1470,8 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_lists(TraceArgs A, cons
     }
     if (A.tile_stats && lane == 0) A.tile_stats[4 * (size_t)(A.tile_base_launch + tile) + 2] = nrounds;
     if (!active) return;
-    store_pixel(A, V, pix, r, depth_on, T, Tf, pnc, skc0, skc1, skc2, ska, C0, C1, C2, D, ncomp, (uint32_t)nskip);
+    store_pixel(A, V, pix, r, depth_on, T, Tf, pnc, skc0, skc1, skc2, ska, C0, C1, C2, D, ncomp,
+                (uint32_t)(traced ? n : 0));
 }
 
 // ---------------------------------------------------------------------------
@@ -2952,6 +2965,8 @@ int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
     }
     if (lanes > ctx->list_lanes_cap) {
         GRAY_HIP(hipFree(ctx->list_n));
+        GRAY_HIP(hipFree(ctx->tile_open));
+        ctx->tile_open = nullptr;
         for (uint2 *&t : ctx->tasks) {
             GRAY_HIP(hipFree(t));
             t = nullptr;
@@ -2961,6 +2976,8 @@ int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
         ctx->task_cap = 0;
         if (hipMalloc(&ctx->list_n, sizeof(int32_t) * lanes) != hipSuccess)
             return fail("list counts", sizeof(int32_t) * lanes);
+        if (hipMalloc(&ctx->tile_open, (size_t)(lanes / GRAY_WAVE)) != hipSuccess)
+            return fail("tile flags", lanes / GRAY_WAVE);
         for (uint2 *&t : ctx->tasks)
             if (hipMalloc(&t, sizeof(uint2) * lanes * kTaskPerLane) != hipSuccess)
                 return fail("task buffers", sizeof(uint2) * lanes * kTaskPerLane);
@@ -3134,6 +3151,7 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             if ((rc = reserve_lists(ctx, lanes, args->list_capacity))) return rc;
             A.list_rec = ctx->list_rec;
             A.list_n = ctx->list_n;
+            A.tile_open = ctx->tile_open;
             A.list_stride = lanes;
             A.list_cap = args->list_capacity;
             A.task_cap = (int32_t)ctx->task_cap;
@@ -3149,6 +3167,13 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             // list has one order
             const bool tasks = !args->deterministic;
             A.budget = !tasks ? -1 : args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
+            // compositing: the task buffers are free once the last round has
+            // run, and the first then holds the queue of rays composited by a
+            // whole wave (one entry per lane)
+            A.heavy_count = ctx->task_counters + 2 * kTaskRounds + 2;
+            A.heavy_rays = reinterpret_cast<uint32_t *>(ctx->tasks[0]);
+            A.wave_list_min = args->wave_list_min == 0 ? GRAY_WAVE_LIST_MIN : args->wave_list_min;
+            A.retrace_any = ctx->task_counters + 2 * kTaskRounds + 3;  // zeroed with the task counters
             if ((rc = mark(0))) return rc;
             const Bvh4Node *cnodes = scene_of(ctx)->wide;
             if (tasks)
@@ -3157,6 +3182,18 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
                 hipLaunchKernelGGL(k_collect<kDetStack>, grid, block, 0, s, A, A.geo, cnodes);
             GRAY_HIP(hipGetLastError());
             if ((rc = mark(1))) return rc;
+            // tiles whose rays handed nothing on have final lists: they are
+            // composited on the side stream (after the colours) beside the
+            // pooled round, which walks only the other tiles' rays
+            const bool early = tasks;
+            bool side_work = false;
+            if (early) {
+                GRAY_HIP(hipEventRecord(ctx->ev_fork, s));
+                GRAY_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+                hipLaunchKernelGGL(k_composite_lists, grid, block, 0, ctx->side, A, A.views[0].rgb, 1);
+                GRAY_HIP(hipGetLastError());
+                side_work = true;
+            }
             // pooled rounds; all but the last may hand work on (budget >= 0)
             for (int round = 0; tasks && round < kTaskRounds; ++round) {
                 A.task_in = ctx->tasks[round & 1];
@@ -3175,12 +3212,6 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             }
             if ((rc = mark(2))) return rc;
             if ((rc = join_sh())) return rc;
-            // the task buffers are free once the last round has run: the first
-            // holds the queue of rays composited by a whole wave (one entry per lane)
-            A.heavy_count = ctx->task_counters + 2 * kTaskRounds + 2;
-            A.heavy_rays = reinterpret_cast<uint32_t *>(ctx->tasks[0]);
-            A.wave_list_min = args->wave_list_min == 0 ? GRAY_WAVE_LIST_MIN : args->wave_list_min;
-            A.retrace_any = ctx->task_counters + 2 * kTaskRounds + 3;  // zeroed with the task counters
             if (A.wave_list_min >= 0) {
                 // the long lists are queued first; their wave-per-ray kernel then
                 // runs on the side stream beside the per-lane kernel (both are
@@ -3191,11 +3222,16 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
                 GRAY_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
                 hipLaunchKernelGGL(k_composite_wave, dim3(kWaveGrid), block, 0, ctx->side, A, A.views[0].rgb);
                 GRAY_HIP(hipGetLastError());
-                GRAY_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+                side_work = true;
             }
-            hipLaunchKernelGGL(k_composite_lists, grid, block, 0, s, A, A.views[0].rgb);
+            hipLaunchKernelGGL(k_composite_lists, grid, block, 0, s, A, A.views[0].rgb, early ? 2 : 0);
             GRAY_HIP(hipGetLastError());
-            if (A.wave_list_min >= 0) GRAY_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+            // joined before the re-traversal, and before the next chunk of views
+            // reuses the lists
+            if (side_work) {
+                GRAY_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+                GRAY_HIP(hipStreamWaitEvent(s, ctx->ev_join, 0));
+            }
         } else {
             if ((rc = join_sh())) return rc;
             for (int i = 0; i < 3; ++i)
