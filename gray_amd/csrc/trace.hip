@@ -1898,6 +1898,17 @@ struct RawTable {
         __syncthreads();
         return n0 + __popcll(b1);
     }
+    // After compact(): every occupied slot's sums straight to the device.
+    __device__ __forceinline__ void flush_global(const TraceArgs &A, const DevView &V, int lane, float pixels,
+                                                 int total) {
+        for (int i = lane; i < total; i += GRAY_WAVE) {
+            const int slot = list[i];
+            float s[kRawTerms];
+#pragma unroll
+            for (int j = 0; j < kRawTerms; ++j) s[j] = val[slot * kRawRow + j];
+            raw_to_global(A, V, key[slot], s, pixels);
+        }
+    }
     // After compact(): every occupied slot's Gaussian, one per lane.  Its
     // sums become record tile * kGradSlots + i, listed under the Gaussian
     // (one integer atomic instead of a float atomic per sum); a Gaussian with
@@ -2136,6 +2147,7 @@ __device__ __forceinline__ bool load_pixel_state(const TraceArgs &A, const DevVi
 // per hit to one per Gaussian.  A Gaussian that found no slot is read from
 // global memory where it occurs and its hits are added after the loop.
 constexpr int kRecRow = 20;  // floats per staged record: W2L 9, mean 3, op, clip, colour 3, pad 3
+constexpr int kSavedFails = 6;  // failed hits per lane redone from saved state instead of a replay
 struct StoredLds {
     uint32_t key[kGradSlots];
     __attribute__((aligned(16))) float val[kGradSlots * kRawRow];
@@ -2221,9 +2233,15 @@ __device__ __forceinline__ void stored_tile(const TraceArgs &A, int tile, Stored
     float gT = 0.0f;
     float Tafter = ps.Tf;  // fp16_hits: transmittances rebuilt from the final one
     // steps whose hit this lane adds itself after the loop (its Gaussian has
-    // no slot); a failure past step 63 drops the whole tile instead
+    // no slot); a failure past step 63 drops the whole tile instead.  The
+    // first kSavedFails of them keep the step and the transmittance gradient
+    // (and final transmittance) they were reached with, so each is redone on
+    // its own; more send the lane through a replay of its list.
     uint64_t failed = 0;
     bool far = false;
+    int nfail = 0;
+    int f_it[kSavedFails];
+    float f_gT[kSavedFails], f_Ta[kSavedFails];
     // list entries of hits k (id0, T0) and k-1 (id1, T1) are in registers or
     // in flight; off2 is the offset of entry k-2, stepped down each step
     uint32_t id0 = GRAY_EMPTY, id1 = GRAY_EMPTY;
@@ -2262,12 +2280,22 @@ __device__ __forceinline__ void stored_tile(const TraceArgs &A, int tile, Stored
                 rx = f4v{oc.x, oc.y, rgb[3 * id0 + 0], rgb[3 * id0 + 1]};
                 c2 = rgb[3 * id0 + 2];
             }
+            const float gT_in = gT, Ta_in = Tafter;
             if (backward_hit_raw(ra, rb, rc, float2{rx[0], rx[1]}, rx[2], rx[3], c2, r, p, ps, T0, gT, rebuild,
                                  Tafter, t)) {
                 if (slot >= 0) {
                     gid = id0;
                 } else if (it < 64) {
                     failed |= 1ull << it;
+#pragma unroll
+                    for (int j = 0; j < kSavedFails; ++j) {
+                        if (nfail == j) {
+                            f_it[j] = it;
+                            f_gT[j] = gT_in;
+                            f_Ta[j] = Ta_in;
+                        }
+                    }
+                    ++nfail;
                 } else {
                     far = true;
                 }
@@ -2326,9 +2354,43 @@ __device__ __forceinline__ void stored_tile(const TraceArgs &A, int tile, Stored
     }
     __syncthreads();
     tab.flush(A, V, lane, pixels, total, tile, vi);
-    // hits whose Gaussian found no slot: replay the list to them, each
-    // straight to the device
-    if (failed) {
+    // Hits whose Gaussian found no slot.  The saved ones are redone from
+    // their state and summed per Gaussian in the cleared table (a Gaussian
+    // that failed is usually hit by many of the tile's rays: one set of
+    // atomics per Gaussian instead of one per hit); a lane with more failures
+    // replays its list, each failed hit straight to the device.
+    const bool saved = failed && nfail <= kSavedFails;
+    int jmax = saved ? nfail : 0;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) jmax = max(jmax, __shfl_xor(jmax, off));
+    if (jmax > 0) {
+        __syncthreads();
+        tab.clear(lane);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kSavedFails; ++j) {
+            if (j >= jmax) break;
+            float t[kRawTerms];
+            int slot = -1;
+            if (saved && j < nfail) {
+                const int k = m - 1 - f_it[j];
+                const uint32_t g = ids[(size_t)k * R];
+                const f4v *g4 = reinterpret_cast<const f4v *>(A.geo + g);
+                float gTj = f_gT[j], Taj = f_Ta[j];
+                if (backward_hit_raw(g4[0], g4[1], g4[2], *reinterpret_cast<const float2 *>(g4 + 3), rgb[3 * g + 0],
+                                     rgb[3 * g + 1], rgb[3 * g + 2], r, p, ps, hT[(size_t)k * R], gTj, rebuild, Taj,
+                                     t)) {
+                    slot = tab.claim(g);
+                    if (slot < 0) raw_to_global(A, V, g, t, pixels);
+                }
+            }
+            tab.add_at(slot, t, lane);
+        }
+        __syncthreads();
+        const int total2 = tab.compact(lane);
+        tab.flush_global(A, V, lane, pixels, total2);
+    }
+    if (failed && !saved) {
         float gT2 = 0.0f, Tafter2 = ps.Tf;
         for (int it = 0; it < m && it < 64 && (failed >> it) != 0; ++it) {
             const int k = m - 1 - it;

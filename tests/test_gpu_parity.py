@@ -217,19 +217,25 @@ def pixel_grid_scene(cam, blocks, logit=0.0):
     (64, [((2.6, 3.0, 3.4), range(64))], 0.0, 16),
     (64, [((2.6, 3.0, 3.4), range(32)), ((3.0,), range(32, 64))], 0.0, 16),
     (16, [(tuple(np.linspace(2.0, 4.0, 70)), range(16))], -3.5, 80),
-], ids=["every-tile", "half", "deep"])
+    (64, [((2.6, 2.9, 3.2, 3.5, 3.8), range(64))], -1.0, 16),
+    (16, [(tuple(np.linspace(2.0, 4.0, 40)), range(16))], -3.5, 48),
+], ids=["every-tile", "half", "deep", "second-table-full", "replay"])
 def test_backward_table_overflow(size, blocks, logit, cap):
     """Tiles whose composited hits span more distinct Gaussians than the
     stored-list backward's per-tile table holds (128 slots): three Gaussians on
-    every pixel's ray make 192 per 8x8 tile: the hits whose Gaussian found no
-    slot (and those merged into them) are added hit by hit after the table's
-    flush.  In the "half" frame the right half has one Gaussian per pixel (64
-    per tile), so tables that overflow and tables that do not share the launch.
-    In the "deep" frame 70 faint Gaussians lie on every ray, so slots run out
-    past the 64 steps a lane can mark: those tiles drop their table and the
-    recompute kernel redoes their stored lists.  Gradients equal the oracle's
-    element by element, on two backward calls in a row (the first clears the
-    marks it leaves)."""
+    every pixel's ray make 192 per 8x8 tile.  The hits whose Gaussian found no
+    slot (at most 6 per lane) are redone after the flush and summed per
+    Gaussian in the cleared table, then added to the device.  In the "half"
+    frame the right half has one Gaussian per pixel (64 per tile), so tables
+    that overflow and tables that do not share the launch.  With five per ray
+    ("second-table-full", 320 per tile) the Gaussians left over do not all fit
+    the second table either, and those hits go to the device one by one.  With
+    40 per ray ("replay") a lane has more failed hits than it keeps, and
+    replays its list to them.  In the "deep" frame 70 faint Gaussians lie on
+    every ray, so slots run out past the 64 steps a lane can mark: those tiles
+    drop their table and the recompute kernel redoes their stored lists.
+    Gradients equal the oracle's element by element, on two backward calls in
+    a row (the first clears the marks it leaves)."""
     cam = scenes.tiny_camera(size, size)
     scene = pixel_grid_scene(cam, blocks, logit)
     nr = setup_native(scene, cam, t_threshold=0.0001, hit_capacity=cap, sh=False, channels=scene.channels())
@@ -239,8 +245,10 @@ def test_backward_table_overflow(size, blocks, logit, cap):
     g = nr.get_gaussians()
     ncomp = nr.hits.n_comp[: size * size].cpu().numpy().reshape(size, size)
     for depths, columns in blocks:
-        # every Gaussian of a short column is composited; a deep column passes step 64
-        assert (ncomp[:, list(columns)] >= min(len(depths), 65)).all(), "grid pixels composite their Gaussians"
+        # every Gaussian of a short column is composited; a deep column passes
+        # step 64; of 40, a few of the faintest may fall under the alpha threshold
+        need = len(depths) - 5 if 10 < len(depths) < 64 else min(len(depths), 65)
+        assert (ncomp[:, list(columns)] >= need).all(), "grid pixels composite their Gaussians"
     gout = torch.randn(fb.grad_output_channels.shape, generator=torch.Generator().manual_seed(4)).cuda()
     fb.grad_output_channels.copy_(gout)
     names = ("grad_mean", "grad_rotation", "grad_scale", "grad_opacity", "grad_channels", "pruning_weight")
