@@ -2147,7 +2147,7 @@ __device__ __forceinline__ bool load_pixel_state(const TraceArgs &A, const DevVi
 // per hit to one per Gaussian.  A Gaussian that found no slot is read from
 // global memory where it occurs and its hits are added after the loop.
 constexpr int kRecRow = 20;  // floats per staged record: W2L 9, mean 3, op, clip, colour 3, pad 3
-constexpr int kSavedFails = 6;  // failed hits per lane redone from saved state instead of a replay
+constexpr int kSavedFails = 12;  // failed hits per lane redone from saved state instead of a replay
 struct StoredLds {
     uint32_t key[kGradSlots];
     __attribute__((aligned(16))) float val[kGradSlots * kRawRow];

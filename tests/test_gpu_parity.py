@@ -224,7 +224,7 @@ def test_backward_table_overflow(size, blocks, logit, cap):
     """Tiles whose composited hits span more distinct Gaussians than the
     stored-list backward's per-tile table holds (128 slots): three Gaussians on
     every pixel's ray make 192 per 8x8 tile.  The hits whose Gaussian found no
-    slot (at most 6 per lane) are redone after the flush and summed per
+    slot (at most 12 per lane) are redone after the flush and summed per
     Gaussian in the cleared table, then added to the device.  In the "half"
     frame the right half has one Gaussian per pixel (64 per tile), so tables
     that overflow and tables that do not share the launch.  With five per ray
