@@ -284,13 +284,21 @@ def run_train(args, traffic):
         step()
     torch.cuda.synchronize()
     nm.device_status()
-    # timed region: exactly K steps
-    timer = PhaseTimer()
-    nm.timer = timer
-    step_events = []
+    # timed region: exactly K steps, with no events inside it (a timing
+    # event on the stream costs the GPU a few microseconds of idle)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nm.device_status()
+    total_s = t1 - t0
+    # untimed: per-step latency (events around each step) and the phases of
+    # the native calls (events around each), over further steps
+    log(f"[bench] per-step and per-phase timing: {args.kernel_reps} steps each")
+    step_events = []
+    for _ in range(args.kernel_reps):
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -298,17 +306,19 @@ def run_train(args, traffic):
         e1.record()
         step_events.append((e0, e1))
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    nm.device_status()
+    step_ms = sorted(a.elapsed_time(b) for a, b in step_events)
+    timer = PhaseTimer()
+    nm.timer = timer
+    for _ in range(args.kernel_reps):
+        step()
     nm.timer = _NoTimer()
     phases = timer.summary()
-    step_ms = sorted(a.elapsed_time(b) for a, b in step_events)
-    total_s = t1 - t0
     ms_per_step = total_s * 1e3 / args.steps
     rays = W * H
-    stats = nm.get_stats()
-    hits = float(stats.num_gaussians_hit.float().sum()) / max(1, (args.warmup + args.steps) * rays)
-    acc = float(stats.num_gaussians_accumulated.float().sum()) / max(1, (args.warmup + args.steps) * rays)
+    stats = nm.get_stats()  # summed over every forward so far
+    frames = args.warmup + args.steps + 2 * args.kernel_reps
+    hits = float(stats.num_gaussians_hit.float().sum()) / max(1, frames * rays)
+    acc = float(stats.num_gaussians_accumulated.float().sum()) / max(1, frames * rays)
 
     # dominant kernel: HIP events recorded by the library around it, on the
     # stream it runs on, over further steps of the same workload
@@ -361,8 +371,12 @@ def run_train(args, traffic):
         log(f"[bench] {args.loss_steps} steps with the L1 + SSIM loss (lambda 0.2, fused loss kernels)")
         step_ls = make_step(rt, cam, target, "l1ssim")
         step_ls()
-        ls_ms, _ = _median_ms(step_ls, args.loss_steps)
-        out["step_l1_ssim_ms_p50"] = round(ls_ms, 4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.loss_steps):
+            step_ls()
+        torch.cuda.synchronize()
+        out["step_l1_ssim_ms"] = round((time.perf_counter() - t0) * 1e3 / args.loss_steps, 4)
     if not args.no_extra:
         # the same step with the deterministic gradient reduction (walks
         # without continuation tasks, per-Gaussian sums in pixel order); not
