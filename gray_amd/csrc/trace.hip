@@ -2189,16 +2189,17 @@ __device__ __forceinline__ void stored_tile(const TraceArgs &A, int tile, Stored
     const float *rgb = V.rgb;
     const bool rebuild = A.fp16 != 0;
     __syncthreads();
-    // 1. a slot for every Gaussian of the tile's lists (entries read back to
-    // front, a step ahead; a Gaussian that finds none is looked up in vain
-    // below and read from global memory)
+    // 1. a slot for every Gaussian of the tile's lists (entries read front
+    // to back, a step ahead: the front hits, on surfaces many of the tile's
+    // rays share, take the slots first when the table fills; a Gaussian that
+    // finds none is looked up in vain below and read from global memory)
     {
-        size_t off = (size_t)(m - 1) * R;
-        uint32_t g = m >= 1 ? ids[off] : GRAY_EMPTY;
-        for (int k = m - 1; k >= 0; --k) {
-            off -= R;
+        size_t off = 0;
+        uint32_t g = m >= 1 ? ids[0] : GRAY_EMPTY;
+        for (int k = 0; k < m; ++k) {
+            off += R;
             const uint32_t cur = g;
-            if (k >= 1) g = ids[off];
+            if (k + 1 < m) g = ids[off];
             tab.claim(cur);
         }
     }
