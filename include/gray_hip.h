@@ -334,18 +334,21 @@ int gray_sh_backward(gray_ctx *ctx, const gray_params *params, const gray_knobs 
  *      ray's list in the context (16 B per hit, slot-major); after a budget
  *      of iterations the rest of a walk is handed on as {ray, subtree} tasks;
  *      a persistent pooled round gives each wave 32 tasks whose frontier
- *      (node and leaf stacks in LDS) its 64 lanes drain together, one node
- *      and one leaf per lane and step, so long walks are split across lanes;
- *   2. composite: one lane per ray scans its list with a 16-entry register
+ *      (node and leaf stacks in LDS) its 64 lanes drain together, two nodes
+ *      and two leaves per lane and step, so long walks are split across lanes;
+ *   2. composite: one lane per ray scans its list with an 8-entry register
  *      k-buffer, in the reference's batches of 32 beyond tmin (sub-rounds of
- *      16), front to back, stopping at T < t_threshold; lists longer than
+ *      8), front to back, stopping at T < t_threshold; lists longer than
  *      wave_list_min (up to 512 hits) are composited by a whole wave each;
+ *      the tiles none of whose rays handed work on are composited while the
+ *      pooled round still runs;
  *   3. rays whose list or tasks overflowed are re-traversed per batch by a
  *      k-buffer kernel (16 entries in LDS), as is every ray when
  *      list_capacity == 0.
  * The context's second stream runs the SH colours (sh_params) beside the
- * collect pass and the wave-per-ray compositing beside the per-lane one; both
- * are joined back into `stream` with events before the call returns.
+ * collect pass, the compositing of the finished tiles beside the pooled
+ * round and the wave-per-ray compositing beside the per-lane one; all are
+ * joined back into `stream` with events before the call returns.
  * Cost model: the collect walk is bound by the dependent latency of its
  * node visits (about 95 4-wide node visits and 44 leaf tests per ray on a
  * 1M-Gaussian surface scene); compositing reads each list once. */
