@@ -1632,6 +1632,20 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_wave(TraceArgs A, const
             d[P - 1] = INFINITY, id[P - 1] = 0xFFFFFFFFu, js[P - 1] = 0u;
             --cnt;
         };
+        // the same for the lanes where `on` holds, as selects in every lane
+        // (no branch: the compiler would copy the whole list on both paths)
+        auto pop_if = [&](bool on) {
+#pragma unroll
+            for (int j = 0; j + 1 < P; ++j) {
+                d[j] = on ? d[j + 1] : d[j];
+                id[j] = on ? id[j + 1] : id[j];
+                js[j] = on ? js[j + 1] : js[j];
+            }
+            d[P - 1] = on ? INFINITY : d[P - 1];
+            id[P - 1] = on ? 0xFFFFFFFFu : id[P - 1];
+            js[P - 1] = on ? 0u : js[P - 1];
+            cnt -= on ? 1 : 0;
+        };
         float T = 1.0f, Tf = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f, D = 0.0f, tmin = znear;
         uint32_t ncomp = 0;
         const bool traced = tmin < zfar;
@@ -1667,7 +1681,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_composite_wave(TraceArgs A, const
                 T = T * (1.0f - av);
                 Tf = Tf * (1.0f - a32);
                 ++ncomp;
-                if (lane == owner) pop();
+                pop_if(lane == owner);
                 if (T < tthr) break;
                 if (++in_round == GRAY_KB) {
                     in_round = 0;
