@@ -74,7 +74,7 @@ class RenderArgs(C.Structure):
         ("knobs", Knobs), ("num_views", i32), ("views", C.POINTER(View)), ("save", i32),
         ("hit_capacity", i32), ("use_bvh", i32), ("tile_stats", vp), ("list_capacity", i32),
         ("collect_budget", i32), ("fp16_hits", i32), ("wave_list_min", i32), ("deterministic", i32),
-        ("sh_params", vp),
+        ("sh_params", vp), ("collect_mode", i32),
     ]
 
 
@@ -150,7 +150,7 @@ def _load():
     return lib
 
 
-ABI_VERSION = 13  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
+ABI_VERSION = 14  # GRAY_ABI_VERSION of include/gray_hip.h that these structures mirror
 LIB = _load()
 if LIB.gray_abi_version() != ABI_VERSION:
     raise ImportError(f"libgray_hip.so has ABI {LIB.gray_abi_version()}, the binding expects {ABI_VERSION}; "

@@ -65,7 +65,8 @@ extern "C" void gray_ctx_destroy(gray_ctx *ctx) {
                     ctx->wide,     ctx->wide_flag,  ctx->det_rec, ctx->det_keys[0], ctx->det_keys[1],
                     ctx->det_vals[0], ctx->det_vals[1], ctx->det_off, ctx->det_hist, ctx->det_sums,
                     ctx->raw,      ctx->raw_touched, ctx->bwd_overflow, ctx->tile_cost, ctx->tile_order,
-                    ctx->tile_next, ctx->bwd_rec, ctx->bwd_cnt, ctx->bwd_lst, ctx->tile_open};
+                    ctx->tile_next, ctx->bwd_rec, ctx->bwd_cnt, ctx->bwd_lst, ctx->tile_open,
+                    ctx->bin_tiles, ctx->bin_items};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);

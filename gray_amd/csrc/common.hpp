@@ -136,6 +136,18 @@ struct TraceArgs {
     int32_t *task_count_in, *task_count_out, *task_head;
     int32_t task_cap;
     int32_t fp16;  // fp16_hits: distances and compositing alphas rounded to fp16 as the reference stores them
+    // tile bins of the collect pass (collect_mode 0): bin_tiles holds per
+    // launch tile its pair count [T], then the fill cursors [T + 1] (after the
+    // fill: the end of the tile's pairs), then the first work item of each tile
+    // [T + 1]; bins the Gaussian ids grouped by tile; bin_items a (tile, chunk)
+    // pair per work item; *bin_mode 0 while the bins serve the launch, 1 when
+    // it falls back to the walk (explicit rays, or more pairs than bin_cap)
+    uint32_t *bins;
+    int32_t *bin_tiles;
+    int2 *bin_items;
+    int32_t *bin_mode;
+    int64_t bin_cap;
+    int32_t bin_item_cap, pad_bins;
     // rays whose list is composited by a whole wave: lists longer than
     // wave_list_min (< 0: none), queued by the per-lane compositing kernel
     int32_t *heavy_count;
@@ -179,14 +191,24 @@ struct TraceArgs {
 // reference's t_local/|d'| with t_local = -o'.(d'/|d'|)), local point
 // x = o' + t d'.  Reject if |x|^2 > 1.  Returns t (the reference "distance").
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ bool gray_hit_test(const float *__restrict__ m, float mux, float muy, float muz,
-                                              float ox, float oy, float oz, float dx, float dy, float dz,
-                                              float &dist, float &hx, float &hy, float &hz) {
+// The ray origin in the Gaussian's local frame, o' = M (o - mu): the first
+// half of the hit test.  Rays that share an origin (a pinhole view) share it.
+__device__ __forceinline__ void gray_local_origin(const float *__restrict__ m, float mux, float muy, float muz,
+                                                  float ox, float oy, float oz, float &lox, float &loy,
+                                                  float &loz) {
 #pragma clang fp contract(off)
     const float px = ox - mux, py = oy - muy, pz = oz - muz;
-    const float lox = (m[0] * px + m[1] * py) + m[2] * pz;
-    const float loy = (m[3] * px + m[4] * py) + m[5] * pz;
-    const float loz = (m[6] * px + m[7] * py) + m[8] * pz;
+    lox = (m[0] * px + m[1] * py) + m[2] * pz;
+    loy = (m[3] * px + m[4] * py) + m[5] * pz;
+    loz = (m[6] * px + m[7] * py) + m[8] * pz;
+}
+
+// The second half, from o': the same operations in the same order, so the
+// split test and gray_hit_test agree bit for bit.
+__device__ __forceinline__ bool gray_hit_test_local(const float *__restrict__ m, float lox, float loy, float loz,
+                                                    float dx, float dy, float dz, float &dist, float &hx,
+                                                    float &hy, float &hz) {
+#pragma clang fp contract(off)
     const float ldx = (m[0] * dx + m[1] * dy) + m[2] * dz;
     const float ldy = (m[3] * dx + m[4] * dy) + m[5] * dz;
     const float ldz = (m[6] * dx + m[7] * dy) + m[8] * dz;
@@ -201,6 +223,14 @@ __device__ __forceinline__ bool gray_hit_test(const float *__restrict__ m, float
     if (sq > 1.0f) return false;
     dist = t;
     return true;
+}
+
+__device__ __forceinline__ bool gray_hit_test(const float *__restrict__ m, float mux, float muy, float muz,
+                                              float ox, float oy, float oz, float dx, float dy, float dz,
+                                              float &dist, float &hx, float &hy, float &hz) {
+    float lox, loy, loz;
+    gray_local_origin(m, mux, muy, muz, ox, oy, oz, lox, loy, loz);
+    return gray_hit_test_local(m, lox, loy, loz, dx, dy, dz, dist, hx, hy, hz);
 }
 
 // Display test of the proxy ellipsoid (the unit sphere in local coordinates),

@@ -55,6 +55,11 @@ struct gray_ctx {
     uint2 *tasks[2] = {nullptr, nullptr};
     int64_t task_cap = 0;
     int32_t *task_counters = nullptr;  // [2 * kMaxTaskRounds + 2]
+    // tile bins of the collect pass (the bins themselves live in tasks[1]):
+    // per tile its pair count, fill cursor and first work item; the work items
+    int32_t *bin_tiles = nullptr;      // [3 * list_lanes_cap / 64 + 2]
+    int2 *bin_items = nullptr;         // [bin_item_cap]
+    int64_t bin_item_cap = 0;
     // deterministic gradient reduction: per-hit records, their sort keys and scratch
     float *det_rec = nullptr;        // [det_cap][16]: 15 gradient terms, Gaussian id bits
     uint32_t *det_keys[2] = {nullptr, nullptr}, *det_vals[2] = {nullptr, nullptr};

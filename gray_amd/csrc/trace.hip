@@ -675,6 +675,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect(TraceArgs A, const GeoRec
                                                        const Bvh4Node *__restrict__ nodes) {
     __shared__ uint32_t s_stack[kStack * GRAY_WAVE];
     __shared__ uint32_t s_leaf[kLeafQueue * GRAY_WAVE];
+    if (A.bin_mode && *A.bin_mode == 0) return;  // the tile bins serve this launch
     int vi, tx, ty, tile;
     if (!block_tile(A, vi, tx, ty, &tile)) return;
     const DevView &V = A.views[vi];
@@ -767,6 +768,7 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_pool(TraceArgs A, const G
     const int lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
     const int total = min(*A.task_count_in, A.task_cap);
+    if (total <= 0) return;  // nothing handed on (or the tile bins served the launch)
     const int cap = A.list_cap;
     const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power;
     float one = 1.0f;
@@ -927,6 +929,451 @@ __global__ __launch_bounds__(GRAY_WAVE) void k_collect_pool(TraceArgs A, const G
             }
             __syncthreads();
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile-binned collect pass (collect_mode 0).  Every ray of a pinhole view
+// leaves the view's origin, so the rays that can meet a Gaussian are those of
+// the pixels its box projects to.  Each Gaussian's box (k_snapshot's, padded
+// past the ellipsoid) is projected into every view of the launch; the (8x8
+// tile, Gaussian) pairs are counted per tile, scanned and filled into the
+// tiles' bins; then the 64 rays of a tile test every Gaussian of its bin with
+// the walk's exact test (gray_local_origin once per Gaussian and view, then
+// gray_hit_test_local per ray: the operations of gray_hit_test), so each
+// list holds the hits the walk finds.  The walk is a dependent chain of node
+// fetches per lane; here a wave tests one staged record at a time with all
+// 64 lanes, so the pass is bound by VALU work (about 50 operations per ray
+// and bin entry).  A launch with explicit rays, a degenerate camera or more
+// pairs than the bins hold sets *bin_mode and the walk runs instead.
+// ---------------------------------------------------------------------------
+constexpr int kBinChunk = 512;     // bin entries per work item (a tile with more is split over several)
+constexpr int kBinThreads = 256;   // threads of the projection kernels (one Gaussian each)
+constexpr int kBinScanThreads = 1024;
+
+// View projection: (a, b, w) = inv (p - origin) gives the pixel coordinates
+// xf = a / w, yf = b / w of make_ray, whose ray direction is
+// B (xf, yf, 1) with B = [c0, c1, -c2] (camera-to-world columns);
+// xf = sx (2 (px + 0.5) / W - 1), yf = sy (1 - 2 (py + 0.5) / H).  fp32: the
+// rounding (about 1e-6 relative) stays far inside the pixel slack below.
+struct BinProj {
+    float inv[9];
+    float sx, sy;
+    float o[3];
+    int w, h, tiles_x, tile_base;
+    bool ok;
+};
+
+__device__ void bin_proj(const DevView &V, BinProj &P) {
+    const float *c = V.c2w;
+    const float b0 = c[0], b1 = c[1], b2 = -c[2];
+    const float b3 = c[3], b4 = c[4], b5 = -c[5];
+    const float b6 = c[6], b7 = c[7], b8 = -c[8];
+    const float k0 = b4 * b8 - b5 * b7, k1 = b5 * b6 - b3 * b8, k2 = b3 * b7 - b4 * b6;
+    const float det = b0 * k0 + b1 * k1 + b2 * k2;
+    const float r = 1.0f / det;
+    P.inv[0] = k0 * r, P.inv[1] = (b2 * b7 - b1 * b8) * r, P.inv[2] = (b1 * b5 - b2 * b4) * r;
+    P.inv[3] = k1 * r, P.inv[4] = (b0 * b8 - b2 * b6) * r, P.inv[5] = (b2 * b3 - b0 * b5) * r;
+    P.inv[6] = k2 * r, P.inv[7] = (b1 * b6 - b0 * b7) * r, P.inv[8] = (b0 * b4 - b1 * b3) * r;
+    // the scales as make_ray rounds them
+    const float vs = tanf(*V.fov / 2.0f);
+    const float aspect = (float)V.width / (float)V.height;
+    P.sx = aspect * vs;
+    P.sy = vs;
+    P.o[0] = V.origin[0], P.o[1] = V.origin[1], P.o[2] = V.origin[2];
+    P.w = V.width, P.h = V.height, P.tiles_x = V.tiles_x, P.tile_base = V.tile_base;
+    // a camera basis far from orthonormal (|det| off 1 by more than 1e-3) is
+    // left to the walk, so the rounding bound above holds
+    bool ok = fabsf(fabsf(det) - 1.0f) < 1e-3f && P.sx > 0.0f && P.sy > 0.0f && isfinite(P.sx) && isfinite(P.sy);
+    for (int i = 0; i < 9; ++i) ok = ok && isfinite(c[i]);
+    for (int i = 0; i < 3; ++i) ok = ok && isfinite(P.o[i]);
+    P.ok = ok;
+}
+
+// Tiles [tx0, tx1] x [ty0, ty1] of the view whose pixels' rays can meet box b
+// (min xyz, max xyz); false when no ray can.  Corners in front of the origin
+// (w > 0) are projected.  Where the box reaches the origin's plane (w = 0),
+// its projection runs off to infinity: along an edge from a front corner
+// (a_f, b_f, w_f) to the plane, (a / w, b / w) moves on a straight line from
+// the corner's projection in the direction of the crossing point (a0, b0),
+// so the rectangle reaches the view's edge on the side of each of its signs
+// (both sides where a coordinate is within rounding of 0; a box around the
+// origin covers the whole view).  Corners within rounding of the plane count
+// as on it.  The pixel bounds get 1.5 pixels of slack: half a pixel of jitter
+// and one for rounding.  Not inlined: the count and fill passes call one
+// compiled copy, so they see the same rectangles (two inlined copies could be
+// contracted differently).
+__device__ __noinline__ bool bin_rect(const BinProj &P, const float *b, int &tx0, int &tx1, int &ty0, int &ty1) {
+    if (!(b[0] <= b[3] && b[1] <= b[4] && b[2] <= b[5])) return false;  // empty (invisible) or NaN box
+    const float lx = b[0] - P.o[0], ly = b[1] - P.o[1], lz = b[2] - P.o[2];
+    const float ex = b[3] - b[0], ey = b[4] - b[1], ez = b[5] - b[2];
+    float base[3], dx[3], dy[3], dz[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        base[k] = (P.inv[3 * k] * lx + P.inv[3 * k + 1] * ly) + P.inv[3 * k + 2] * lz;
+        dx[k] = P.inv[3 * k] * ex;
+        dy[k] = P.inv[3 * k + 1] * ey;
+        dz[k] = P.inv[3 * k + 2] * ez;
+    }
+    const float inf = __builtin_huge_valf();
+    float xlo = inf, xhi = -inf, ylo = inf, yhi = -inf;
+    auto extend = [&](float a, float c, float tol) {
+        if (a > tol) xhi = inf;
+        else if (a < -tol) xlo = -inf;
+        else xlo = -inf, xhi = inf;
+        if (c > tol) yhi = inf;
+        else if (c < -tol) ylo = -inf;
+        else ylo = -inf, yhi = inf;
+    };
+    float ca[8], cb[8], cw[8], ct[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        float v[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            v[k] = base[k] + ((i & 1) ? dx[k] : 0.0f) + ((i & 2) ? dy[k] : 0.0f) + ((i & 4) ? dz[k] : 0.0f);
+        ca[i] = v[0], cb[i] = v[1], cw[i] = v[2];
+        // rounding of (a, b, w): a few 1e-7 of |v| and of the box's size
+        ct[i] = 1e-5f * (fabsf(v[0]) + fabsf(v[1]) + fabsf(v[2]) + fabsf(lx) + fabsf(ly) + fabsf(lz) + ex + ey + ez) +
+                1e-30f;
+        if (cw[i] > ct[i]) {
+            const float x = ca[i] / cw[i], y = cb[i] / cw[i];
+            xlo = fminf(xlo, x), xhi = fmaxf(xhi, x), ylo = fminf(ylo, y), yhi = fmaxf(yhi, y);
+        } else if (cw[i] >= -ct[i]) {
+            extend(ca[i], cb[i], ct[i]);  // a corner on the plane
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+        // edge e: axis bit k = 1 << (e / 4), i the corner without it
+        const int k = 1 << (e >> 2);
+        const int lowb = e & 3;
+        const int i = k == 1 ? (lowb << 1) : k == 2 ? ((lowb & 1) | ((lowb & 2) << 1)) : lowb;
+        const int j = i | k;
+        const bool fi = cw[i] > ct[i], fj = cw[j] > ct[j];
+        if (fi == fj) continue;
+        const int f = fi ? i : j, q = fi ? j : i;
+        if (cw[q] >= -ct[q]) continue;  // ends on the plane: its corner extended already
+        const float lam = cw[f] / (cw[f] - cw[q]);
+        extend(ca[f] + lam * (ca[q] - ca[f]), cb[f] + lam * (cb[q] - cb[f]), fmaxf(ct[f], ct[q]));
+    }
+    if (!(xlo <= xhi) || !(ylo <= yhi)) return false;  // nothing in front of the origin
+    const float hw = 0.5f * (float)P.w, hh = 0.5f * (float)P.h;
+    const float x0 = floorf((xlo / P.sx + 1.0f) * hw - 2.0f), x1 = ceilf((xhi / P.sx + 1.0f) * hw + 1.0f);
+    const float y0 = floorf((1.0f - yhi / P.sy) * hh - 2.0f), y1 = ceilf((1.0f - ylo / P.sy) * hh + 1.0f);
+    if (!(x1 >= 0.0f && y1 >= 0.0f && x0 <= (float)(P.w - 1) && y0 <= (float)(P.h - 1))) return false;
+    tx0 = (x0 <= 0.0f ? 0 : (int)x0) / GRAY_TILE;
+    tx1 = (x1 >= (float)(P.w - 1) ? P.w - 1 : (int)x1) / GRAY_TILE;
+    ty0 = (y0 <= 0.0f ? 0 : (int)y0) / GRAY_TILE;
+    ty1 = (y1 >= (float)(P.h - 1) ? P.h - 1 : (int)y1) / GRAY_TILE;
+    return true;
+}
+
+// Clears the tiles' pair counts, the lists' counts, the tile flags and the
+// statistics, and decides the launch's mode (thread 0).
+__global__ void k_bin_setup(TraceArgs A, int64_t lanes) {
+    const int T = A.total_tiles;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < T) {
+        A.bin_tiles[i] = 0;
+        A.tile_open[i] = 0;
+    }
+    if (i < lanes) A.list_n[i] = 0;
+    if (A.tile_stats && i < 4 * (int64_t)T) A.tile_stats[4 * (int64_t)A.tile_base_launch + i] = 0;
+    if (i == 0) {
+        int mode = 0;
+        for (int v = 0; v < A.num_views; ++v) {
+            const DevView &V = A.views[v];
+            if (V.rays_d && (!V.rays_en || *V.rays_en)) mode = 1;  // explicit rays: no common origin
+            BinProj P;
+            bin_proj(V, P);
+            if (!P.ok) mode = 1;
+        }
+        *A.bin_mode = mode;
+    }
+}
+
+// Pairs of the Gaussians of one workgroup, counted (fill 0) or written
+// (fill 1).  The workgroup takes 256 consecutive leaves of the BVH's Morton
+// order, so its Gaussians lie close together and project to few tiles: pairs
+// are first counted per tile in an LDS hash table, and the table's tiles then
+// take their counts (count) or reserve their ranges (fill) with one global
+// atomic each instead of one per pair (3.9M pairs on 8.5K tiles serialised on
+// the tiles' counters).  A Gaussian covering more than kBinBigRect tiles (a box
+// reaching the camera's plane) adds its pairs with global atomics, in both
+// kernels; when the table fills, the count kernel does the same for the pairs
+// that found no slot and the fill kernel writes the whole workgroup's pairs
+// that way.  Count and fill compute the same rectangles, so the fill stays
+// inside each tile's scanned range.
+constexpr int kBinHash = 2048;      // LDS hash slots per workgroup
+constexpr int kBinProbes = 16;
+constexpr int kBinBigRect = 256;    // tiles above which a Gaussian's pairs bypass the table
+
+__device__ __forceinline__ uint32_t bin_hash(int tile) { return ((uint32_t)tile * 2654435761u) >> 21; }
+
+__global__ __launch_bounds__(kBinThreads) void k_bin_pairs(TraceArgs A, const float *__restrict__ aabb,
+                                                           const uint32_t *__restrict__ order, int n, int fill) {
+    __shared__ BinProj s_proj[GRAY_MAX_VIEWS];
+    __shared__ int32_t s_key[kBinHash], s_cnt[kBinHash], s_base[kBinHash];
+    __shared__ int32_t s_full;
+    if (*A.bin_mode != 0) return;
+    const int t = threadIdx.x;
+    if (t < A.num_views) bin_proj(A.views[t], s_proj[t]);
+    for (int i = t; i < kBinHash; i += kBinThreads) s_key[i] = -1, s_cnt[i] = 0;
+    if (t == 0) s_full = 0;
+    __syncthreads();
+    const int k = blockIdx.x * kBinThreads + t;
+    const uint32_t g = k < n ? order[k] : 0u;
+    float box[6] = {1.0f, 1.0f, 1.0f, 0.0f, 0.0f, 0.0f};  // empty
+    if (k < n) {
+        const float2 *q = reinterpret_cast<const float2 *>(aabb + 6 * (size_t)g);
+        const float2 p0 = q[0], p1 = q[1], p2 = q[2];
+        box[0] = p0.x, box[1] = p0.y, box[2] = p1.x, box[3] = p1.y, box[4] = p2.x, box[5] = p2.y;
+    }
+    int32_t *cnt = A.bin_tiles, *cur = A.bin_tiles + A.total_tiles;
+    // pass 1: the workgroup's pairs per tile in the table
+    for (int v = 0; v < A.num_views; ++v) {
+        const BinProj &P = s_proj[v];
+        int tx0, tx1, ty0, ty1;
+        if (!bin_rect(P, box, tx0, tx1, ty0, ty1)) continue;
+        const bool big = (tx1 - tx0 + 1) * (ty1 - ty0 + 1) > kBinBigRect;
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const int tile = P.tile_base + ty * P.tiles_x + tx;
+                if (big) {
+                    if (fill) A.bins[atomicAdd(cur + tile, 1)] = g;
+                    else atomicAdd(cnt + tile, 1);
+                    continue;
+                }
+                uint32_t h = bin_hash(tile);
+                int slot = -1;
+                for (int pr = 0; pr < kBinProbes; ++pr, h = (h + 1) & (kBinHash - 1)) {
+                    const int old = atomicCAS(&s_key[h], -1, tile);
+                    if (old == -1 || old == tile) {
+                        slot = (int)h;
+                        break;
+                    }
+                }
+                if (slot >= 0) atomicAdd(&s_cnt[slot], 1);
+                else if (fill) s_full = 1;
+                else atomicAdd(cnt + tile, 1);
+            }
+    }
+    __syncthreads();
+    const bool full = fill && s_full != 0;
+    // pass 2: one global atomic per tile of the table
+    if (!full) {
+        for (int i = t; i < kBinHash; i += kBinThreads) {
+            const int tile = s_key[i];
+            if (tile < 0) continue;
+            if (fill) {
+                s_base[i] = atomicAdd(cur + tile, s_cnt[i]);
+                s_cnt[i] = 0;
+            } else {
+                atomicAdd(cnt + tile, s_cnt[i]);
+            }
+        }
+    }
+    if (!fill) return;
+    __syncthreads();
+    // pass 3 (fill): every table pair takes a rank in its tile's range
+    for (int v = 0; v < A.num_views; ++v) {
+        const BinProj &P = s_proj[v];
+        int tx0, tx1, ty0, ty1;
+        if (!bin_rect(P, box, tx0, tx1, ty0, ty1)) continue;
+        if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > kBinBigRect) continue;  // written in pass 1
+        for (int ty = ty0; ty <= ty1; ++ty)
+            for (int tx = tx0; tx <= tx1; ++tx) {
+                const int tile = P.tile_base + ty * P.tiles_x + tx;
+                if (full) {
+                    A.bins[atomicAdd(cur + tile, 1)] = g;
+                    continue;
+                }
+                uint32_t h = bin_hash(tile);
+                int pr = 0;
+                while (s_key[h] != tile && ++pr < kBinProbes) h = (h + 1) & (kBinHash - 1);  // inserted in pass 1
+                if (s_key[h] == tile) A.bins[s_base[h] + atomicAdd(&s_cnt[h], 1)] = g;
+            }
+    }
+}
+
+// One workgroup: exclusive scans of the pair counts (the fill cursors) and of
+// the work items per tile (ceil(count / kBinChunk)); totals past the bins or
+// the item buffer send the launch to the walk.
+__global__ __launch_bounds__(kBinScanThreads) void k_bin_scan(TraceArgs A) {
+    __shared__ int64_t s_p[kBinScanThreads], s_i[kBinScanThreads];
+    if (*A.bin_mode != 0) return;
+    const int T = A.total_tiles, t = threadIdx.x;
+    const int per = (T + kBinScanThreads - 1) / kBinScanThreads;
+    const int lo = min(T, t * per), hi = min(T, lo + per);
+    const int32_t *cnt = A.bin_tiles;
+    int32_t *cur = A.bin_tiles + T, *ist = A.bin_tiles + 2 * T + 1;
+    int64_t sp = 0, si = 0;
+    for (int i = lo; i < hi; ++i) {
+        const int c = cnt[i];
+        sp += c;
+        si += (c + kBinChunk - 1) / kBinChunk;
+    }
+    s_p[t] = sp;
+    s_i[t] = si;
+    __syncthreads();
+    for (int off = 1; off < kBinScanThreads; off <<= 1) {
+        const int64_t a = t >= off ? s_p[t - off] : 0, b = t >= off ? s_i[t - off] : 0;
+        __syncthreads();
+        s_p[t] += a;
+        s_i[t] += b;
+        __syncthreads();
+    }
+    int64_t p = s_p[t] - sp, it = s_i[t] - si;
+    const bool fits = s_p[kBinScanThreads - 1] <= A.bin_cap && s_i[kBinScanThreads - 1] <= A.bin_item_cap;
+    for (int i = lo; i < hi && fits; ++i) {
+        const int c = cnt[i];
+        cur[i] = (int32_t)p;
+        ist[i] = (int32_t)it;
+        p += c;
+        it += (c + kBinChunk - 1) / kBinChunk;
+    }
+    if (t == kBinScanThreads - 1) {
+        ist[T] = fits ? (int32_t)s_i[t] : 0;
+        if (!fits) *A.bin_mode = 1;
+    }
+}
+
+// One thread per tile: its work items (tile, chunk), tiles in order.
+__global__ void k_bin_items(TraceArgs A) {
+    if (*A.bin_mode != 0) return;
+    const int T = A.total_tiles;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= T) return;
+    const int c = A.bin_tiles[i], first = A.bin_tiles[2 * T + 1 + i];
+    for (int k = 0; k * kBinChunk < c; ++k) A.bin_items[first + k] = make_int2(i, k);
+}
+
+// One wave per work item: the 64 rays of the item's tile test up to kBinChunk
+// entries of its bin, 64 at a time.  The entries' records are staged in LDS
+// by one lane each (with o' = M (o - mu) for the view's origin, and those
+// below the alpha threshold dropped); every lane then tests its ray against
+// each staged record in turn (a broadcast read), keeping a hit mask, and
+// writes the hits after the chunk: at the tile's own running count when the
+// bin is one item, else at slots taken with one atomic per lane and chunk
+// (items of one tile run concurrently).  Items are mapped XCD-aware: XCD k
+// takes the k-th eighth of the items, in tile order, so the neighbouring
+// tiles whose bins share Gaussians share one L2.
+__device__ __forceinline__ void collect_bin_item(const TraceArgs &A, const GeoRecord *__restrict__ geo,
+                                                 float4 *__restrict__ s_rec, int item) {
+    const int T = A.total_tiles;
+    const int2 wi = A.bin_items[item];
+    const int tile = wi.x, chunk = wi.y;
+    int v = 0;
+#pragma unroll
+    for (int k = 1; k < GRAY_MAX_VIEWS; ++k)
+        if (k < A.num_views && tile >= A.views[k].tile_base) v = k;
+    const DevView &V = A.views[v];
+    const int lt = tile - V.tile_base;
+    const int lane = threadIdx.x;
+    const int x = (lt % V.tiles_x) * GRAY_TILE + (lane & 7), y = (lt / V.tiles_x) * GRAY_TILE + (lane >> 3);
+    LaneRay r{};
+    bool active = false;
+    if (x < V.width && y < V.height) active = make_ray(A.knobs, V, x, y, r);
+    const size_t g = (size_t)tile * GRAY_WAVE + lane;
+    const int cnt_t = A.bin_tiles[tile], end_t = A.bin_tiles[T + tile];
+    const int j0 = end_t - cnt_t + chunk * kBinChunk, j1 = min(end_t, j0 + kBinChunk);
+    const bool single = cnt_t <= kBinChunk;
+    const int cap = A.list_cap;
+    const float athr = *A.knobs.alpha_threshold, p = *A.knobs.exp_power;
+    const float ox = V.origin[0], oy = V.origin[1], oz = V.origin[2];
+    const uint64_t below = (1ull << lane) - 1ull;
+    int n = 0, tests = 0;
+    bool ok = true;
+    // the next chunk's entry and record are requested before the current
+    // chunk's tests, so their loads overlap the tests
+    uint32_t gid = j0 + lane < j1 ? A.bins[j0 + lane] : GRAY_EMPTY;
+    bool valid = gid < (uint32_t)A.num_gaussians;
+    GeoV q = valid ? load_geo(geo, gid) : GeoV{};
+    for (int j = j0; j < j1; j += GRAY_WAVE) {
+        bool keep = false;
+        float4 q0, q1, q2, q3;
+        if (valid) {
+            keep = q.op > athr;
+            float lox, loy, loz;
+            gray_local_origin(q.m, q.mu[0], q.mu[1], q.mu[2], ox, oy, oz, lox, loy, loz);
+            q0 = make_float4(q.m[0], q.m[1], q.m[2], q.m[3]);
+            q1 = make_float4(q.m[4], q.m[5], q.m[6], q.m[7]);
+            q2 = make_float4(q.m[8], lox, loy, loz);
+            q3 = make_float4(q.op, q.clip, __uint_as_float(gid), 0.0f);
+        }
+        const uint64_t kb = __ballot(keep);
+        if (keep) {
+            float4 *d = s_rec + 4 * __popcll(kb & below);
+            d[0] = q0, d[1] = q1, d[2] = q2, d[3] = q3;
+        }
+        const int mk = __popcll(kb);
+        __syncthreads();  // the staged records are visible to every lane
+        {
+            const int jn = j + GRAY_WAVE + lane;
+            gid = jn < j1 ? A.bins[jn] : GRAY_EMPTY;
+            valid = gid < (uint32_t)A.num_gaussians;
+            if (valid) q = load_geo(geo, gid);
+        }
+        uint64_t mask = 0;
+        if (active) {
+            for (int k = 0; k < mk; ++k) {
+                const float4 a = s_rec[4 * k], c = s_rec[4 * k + 1], e = s_rec[4 * k + 2];
+                const float mm[9] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w, e.x};
+                float dist, hx, hy, hz;
+                if (gray_hit_test_local(mm, e.y, e.z, e.w, r.dx, r.dy, r.dz, dist, hx, hy, hz)) mask |= 1ull << k;
+            }
+            tests += mk;
+        }
+        const int nh = __popcll(mask);
+        int base = n;
+        if (!single && nh > 0) base = atomicAdd(A.list_n + g, nh);
+        for (int i = 0; mask; ++i) {
+            const int k = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            const float4 a = s_rec[4 * k], c = s_rec[4 * k + 1], e = s_rec[4 * k + 2], f = s_rec[4 * k + 3];
+            const float mm[9] = {a.x, a.y, a.z, a.w, c.x, c.y, c.z, c.w, e.x};
+            float dist = 0.0f, hx = 0.0f, hy = 0.0f, hz = 0.0f, gv, qq;
+            gray_hit_test_local(mm, e.y, e.z, e.w, r.dx, r.dy, r.dz, dist, hx, hy, hz);
+            const float al = gray_alpha(hx, hy, hz, f.y, f.x, p, gv, qq);
+            dist = fp16_round(dist, A.fp16 != 0);
+            const int slot = base + i;
+            if (slot >= 0 && slot < cap) put_hit(A, g, slot, dist, __float_as_uint(f.z), al);
+            else ok = false;
+        }
+        n += nh;
+        __syncthreads();  // every lane has read the records before the next chunk is staged
+    }
+    if (single) {
+        if (active) A.list_n[g] = ok ? n : -1;
+    } else if (!ok) {
+        atomicExch(A.list_n + g, kRayDead);
+    }
+    if (A.tile_stats) {
+        int hs = active ? n : 0, ts = tests;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            hs += __shfl_xor(hs, off);
+            ts += __shfl_xor(ts, off);
+        }
+        if (lane == 0) {
+            int32_t *st = A.tile_stats + 4 * (size_t)(A.tile_base_launch + tile);
+            atomicAdd(st + 1, ts);
+            atomicAdd(st + 3, hs);
+        }
+    }
+}
+
+__global__ __launch_bounds__(GRAY_WAVE) void k_collect_binned(TraceArgs A, const GeoRecord *__restrict__ geo) {
+    __shared__ float4 s_rec[GRAY_WAVE * 4];
+    if (*A.bin_mode != 0) return;
+    const int T = A.total_tiles;
+    const int items = A.bin_tiles[3 * T + 1];
+    const int per = (items + 7) / 8, b = blockIdx.x, stride = gridDim.x >> 3;
+    for (int s8 = b >> 3; s8 < per; s8 += stride) {
+        const int item = (b & 7) * per + s8;
+        if (item >= items) break;
+        collect_bin_item(A, geo, s_rec, item);
     }
 }
 
@@ -3016,9 +3463,11 @@ int fill_args(gray_ctx *ctx, const gray_render_args *args, int first_view, int n
 // Grows the collect-pass lists to lanes * cap records and the task buffers
 // to kTaskPerLane entries per lane (never shrinks).
 constexpr int64_t kTaskPerLane = 16;
+// Tile bins reuse the second task buffer: Gaussian ids, two per task entry.
+inline int64_t bin_capacity(int64_t task_cap) { return std::min<int64_t>(2 * task_cap, INT32_MAX); }
 // device counters of a launch: tasks handed to each round, each round's take
 // position, the wave-per-ray queue length and a spare
-constexpr int kCounters = 2 * kTaskRounds + 4;
+constexpr int kCounters = 2 * kTaskRounds + 5;  // + the tile bins' mode flag
 constexpr int kWaveGrid = 8192;  // persistent waves of the wave-per-ray kernel
 
 int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
@@ -3044,6 +3493,11 @@ int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
         GRAY_HIP(hipFree(ctx->list_n));
         GRAY_HIP(hipFree(ctx->tile_open));
         ctx->tile_open = nullptr;
+        GRAY_HIP(hipFree(ctx->bin_tiles));
+        GRAY_HIP(hipFree(ctx->bin_items));
+        ctx->bin_tiles = nullptr;
+        ctx->bin_items = nullptr;
+        ctx->bin_item_cap = 0;
         for (uint2 *&t : ctx->tasks) {
             GRAY_HIP(hipFree(t));
             t = nullptr;
@@ -3058,6 +3512,14 @@ int reserve_lists(gray_ctx *ctx, int64_t lanes, int cap) {
         for (uint2 *&t : ctx->tasks)
             if (hipMalloc(&t, sizeof(uint2) * lanes * kTaskPerLane) != hipSuccess)
                 return fail("task buffers", sizeof(uint2) * lanes * kTaskPerLane);
+        const int64_t tiles = lanes / GRAY_WAVE;
+        if (hipMalloc(&ctx->bin_tiles, sizeof(int32_t) * (3 * tiles + 2)) != hipSuccess)
+            return fail("tile bin counters", sizeof(int32_t) * (3 * tiles + 2));
+        // work items: at most one partial chunk per tile plus one per full chunk of the bins (tasks[1])
+        const int64_t items = tiles + bin_capacity(lanes * kTaskPerLane) / kBinChunk + 1;
+        if (hipMalloc(&ctx->bin_items, sizeof(int2) * items) != hipSuccess)
+            return fail("tile bin work items", sizeof(int2) * items);
+        ctx->bin_item_cap = items;
         ctx->list_lanes_cap = lanes;
         ctx->task_cap = lanes * kTaskPerLane;
     }
@@ -3243,6 +3705,9 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             // 64-entry stack (an overflow sends the ray to re-traversal), so each
             // list has one order
             const bool tasks = !args->deterministic;
+            // tile bins (collect_mode 0) for pinhole views; the walk below runs
+            // instead when the bins' set-up kernel or scan sets *bin_mode
+            const bool binned = tasks && args->collect_mode == 0;
             A.budget = !tasks ? -1 : args->collect_budget > 0 ? args->collect_budget : GRAY_COLLECT_BUDGET;
             // compositing: the task buffers are free once the last round has
             // run, and the first then holds the queue of rays composited by a
@@ -3253,6 +3718,28 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             A.retrace_any = ctx->task_counters + 2 * kTaskRounds + 3;  // zeroed with the task counters
             if ((rc = mark(0))) return rc;
             const Bvh4Node *cnodes = scene_of(ctx)->wide;
+            if (binned) {
+                A.bins = reinterpret_cast<uint32_t *>(ctx->tasks[1]);
+                A.bin_cap = bin_capacity(ctx->task_cap);
+                A.bin_tiles = ctx->bin_tiles;
+                A.bin_items = ctx->bin_items;
+                A.bin_item_cap = (int32_t)std::min<int64_t>(ctx->bin_item_cap, INT32_MAX);
+                A.bin_mode = ctx->task_counters + 2 * kTaskRounds + 4;  // zeroed with the task counters
+                const int T = A.total_tiles, n = A.num_gaussians;
+                const float *aabb = scene_of(ctx)->aabb;
+                hipLaunchKernelGGL(k_bin_setup, dim3(gray_div_up(std::max<int64_t>(lanes, 4 * (int64_t)T), 256)),
+                                   dim3(256), 0, s, A, lanes);
+                const uint32_t *order = scene_of(ctx)->sorted_ids;  // leaves in Morton order
+                hipLaunchKernelGGL(k_bin_pairs, dim3(gray_div_up(n, kBinThreads)), dim3(kBinThreads), 0, s, A, aabb,
+                                   order, n, 0);
+                hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(kBinScanThreads), 0, s, A);
+                hipLaunchKernelGGL(k_bin_pairs, dim3(gray_div_up(n, kBinThreads)), dim3(kBinThreads), 0, s, A, aabb,
+                                   order, n, 1);
+                hipLaunchKernelGGL(k_bin_items, dim3(gray_div_up(T, 256)), dim3(256), 0, s, A);
+                hipLaunchKernelGGL(k_collect_binned, dim3(8 * std::min(gray_div_up(A.bin_item_cap, 8), 4096u)), block, 0, s, A,
+                                   A.geo);
+                GRAY_HIP(hipGetLastError());
+            }
             if (tasks)
                 hipLaunchKernelGGL(k_collect<kCollectStack>, grid, block, 0, s, A, A.geo, cnodes);
             else
@@ -3262,7 +3749,7 @@ extern "C" int gray_render_forward(gray_ctx *ctx, const gray_render_args *args, 
             // tiles whose rays handed nothing on have final lists: they are
             // composited on the side stream (after the colours) beside the
             // pooled round, which walks only the other tiles' rays
-            const bool early = tasks;
+            const bool early = tasks && !binned;
             bool side_work = false;
             if (early) {
                 GRAY_HIP(hipEventRecord(ctx->ev_fork, s));

@@ -336,6 +336,9 @@ class NativeRaytracer:
         # reproducible gradients: walks without continuation tasks and
         # per-Gaussian sums in pixel order instead of atomics (slower)
         self.deterministic = False
+        # collect pass: 0 = tile bins for pinhole views (BVH walk for explicit
+        # rays), 1 = the BVH walk for every view
+        self.collect_mode = 0
         self.tile_stats = None  # optional int32 [tiles, 4] traversal statistics of the forward
         self.timer = _NoTimer()
         self._built_count = -1
@@ -399,6 +402,7 @@ class NativeRaytracer:
         a.fp16_hits = 1 if self.fp16_hits else 0
         a.wave_list_min = int(self.wave_list_min)
         a.deterministic = 1 if self.deterministic else 0
+        a.collect_mode = int(self.collect_mode)
         return a
 
     def _require_bvh(self) -> None:

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define GRAY_ABI_VERSION 13
+#define GRAY_ABI_VERSION 14
 
 #define GRAY_OK 0
 #define GRAY_ERR_INVALID 1  /* bad argument (null pointer, size, shape) */
@@ -155,6 +155,15 @@ typedef struct {
                                    collect pass runs, and waits for them before compositing; the reference's
                                    forward_pass evaluates SH inside the pass as well (raytracer.h:95-97).
                                    NULL = views[].rgb already holds the colours */
+    int32_t collect_mode;       /* how the collect pass (list_capacity > 0) finds each ray's hits:
+                                   0 = tile bins: every Gaussian's box is projected into each pinhole view,
+                                   the (8x8 tile, Gaussian) pairs are counting-sorted by tile, and each
+                                   tile's rays test its bin (records staged in LDS, the same exact hit test
+                                   as the walk, so the same hits); a launch with explicit rays
+                                   (rays_direction enabled) or more pairs than the task buffers hold falls
+                                   back to the walk on the device.  1 = the BVH walk (per-lane 4-wide walk
+                                   plus pooled task rounds).  The deterministic mode always walks.  No
+                                   counterpart in the reference (OptiX traverses its BVH) */
 } gray_render_args;
 
 /* Gradient targets of the backward (reference gaussians.h:34-62). */

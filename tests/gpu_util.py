@@ -12,7 +12,7 @@ from gray_amd.native import NativeRaytracer
 
 def setup_native(scene, cam: CameraInfo, alpha_threshold=0.01, t_threshold=0.03, exp_power=2.0, bg=(0, 0, 0),
                  sh=True, sh_degree=3, channels=None, hit_capacity=None, znear=0.0, zfar=99999.9,
-                 list_capacity=None, collect_budget=0, wave_list_min=0):
+                 list_capacity=None, collect_budget=0, wave_list_min=0, collect_mode=0):
     n = scene.count if scene is not None else 0
     W, H = cam.image_width, cam.image_height
     nr = NativeRaytracer(W, H, n, 3, 300_000_000, 120_000_000, hit_capacity=hit_capacity)
@@ -20,6 +20,7 @@ def setup_native(scene, cam: CameraInfo, alpha_threshold=0.01, t_threshold=0.03,
         nr.list_capacity = list_capacity
     nr.collect_budget = collect_budget
     nr.wave_list_min = wave_list_min
+    nr.collect_mode = collect_mode
     g = nr.get_gaussians()
     cfg = nr.get_config()
     cfg.alpha_threshold.fill_(alpha_threshold)

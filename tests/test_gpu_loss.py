@@ -91,6 +91,11 @@ def test_raytracer_fused_backward_equals_autograd_path():
     grads = []
     for fused in (False, True):
         rt = Raytracer(RaytracerConfig(), scene.count, 72, 56)
+        # the BVH walk hands no work on at this size, so both instances' lists
+        # hold their hits in one order and the renders agree bit for bit (the
+        # tile bins are filled in atomic order: equal hits, but the skip-set
+        # sums may round differently)
+        rt.cuda_module.collect_mode = 1
         g = rt.cuda_module.get_gaussians()
         with torch.no_grad():
             for name in ("mean", "rotation", "scale", "opacity"):

@@ -21,14 +21,17 @@ pytestmark = pytest.mark.gpu
 # re-traversal while their neighbours composite from lists (3), and lists
 # whose walks stop every 2 iterations so that every ray's remaining subtrees
 # are finished by the task rounds
-@pytest.fixture(scope="module", params=[(None, 0), (0, 0), (3, 0), (None, 2)],
-                ids=["lists", "traverse", "mixed", "tasks"])
+# lists: tile bins; walk / tasks: the BVH walk (tasks: budget 2, so most work
+# goes through the pooled task rounds); mixed: lists of 3 (most rays overflow
+# to re-traversal); traverse: no lists
+@pytest.fixture(scope="module", params=[(None, 0, 0), (0, 0, 0), (3, 0, 0), (None, 0, 1), (None, 2, 1)],
+                ids=["lists", "traverse", "mixed", "walk", "tasks"])
 def tiny(request):
     torch.manual_seed(0)
     scene = scenes.tiny_scene(1024, seed=0)
     cam = scenes.tiny_camera(64, 64)
-    cap, budget = request.param
-    nr = setup_native(scene, cam, list_capacity=cap, collect_budget=budget)
+    cap, budget, mode = request.param
+    nr = setup_native(scene, cam, list_capacity=cap, collect_budget=budget, collect_mode=mode)
     torch.set_grad_enabled(True)
     nr.forward_pass()
     torch.cuda.synchronize()
