@@ -959,6 +959,7 @@ constexpr int kBinScanThreads = 1024;
 struct BinProj {
     float inv[9];
     float sx, sy;
+    float margin;  // pixels of slack around the projected box
     float o[3];
     int w, h, tiles_x, tile_base;
     bool ok;
@@ -999,10 +1000,11 @@ __device__ void bin_proj(const DevView &V, BinProj &P) {
 // so the rectangle reaches the view's edge on the side of each of its signs
 // (both sides where a coordinate is within rounding of 0; a box around the
 // origin covers the whole view).  Corners within rounding of the plane count
-// as on it.  The pixel bounds get 1.5 pixels of slack: half a pixel of jitter
-// and one for rounding.  Not inlined: the count and fill passes call one
+// as on it.  Pixels count whose ray (through the pixel's centre, or within
+// half a pixel of it with jitter) lies within P.margin of the range.  Not inlined: the count and fill passes call one
 // compiled copy, so they see the same rectangles (two inlined copies could be
 // contracted differently).
+
 __device__ __noinline__ bool bin_rect(const BinProj &P, const float *b, int &tx0, int &tx1, int &ty0, int &ty1) {
     if (!(b[0] <= b[3] && b[1] <= b[4] && b[2] <= b[5])) return false;  // empty (invisible) or NaN box
     const float lx = b[0] - P.o[0], ly = b[1] - P.o[1], lz = b[2] - P.o[2];
@@ -1058,9 +1060,12 @@ __device__ __noinline__ bool bin_rect(const BinProj &P, const float *b, int &tx0
         extend(ca[f] + lam * (ca[q] - ca[f]), cb[f] + lam * (cb[q] - cb[f]), fmaxf(ct[f], ct[q]));
     }
     if (!(xlo <= xhi) || !(ylo <= yhi)) return false;  // nothing in front of the origin
-    const float hw = 0.5f * (float)P.w, hh = 0.5f * (float)P.h;
-    const float x0 = floorf((xlo / P.sx + 1.0f) * hw - 2.0f), x1 = ceilf((xhi / P.sx + 1.0f) * hw + 1.0f);
-    const float y0 = floorf((1.0f - yhi / P.sy) * hh - 2.0f), y1 = ceilf((1.0f - ylo / P.sy) * hh + 1.0f);
+    // pixel x's ray passes through continuous coordinate x (+ its jitter):
+    // the pixels within the margin of the projected range
+    const float hw = 0.5f * (float)P.w, hh = 0.5f * (float)P.h, m = P.margin;
+    const float x0 = ceilf((xlo / P.sx + 1.0f) * hw - 0.5f - m), x1 = floorf((xhi / P.sx + 1.0f) * hw - 0.5f + m);
+    const float y0 = ceilf((1.0f - yhi / P.sy) * hh - 0.5f - m), y1 = floorf((1.0f - ylo / P.sy) * hh - 0.5f + m);
+    if (!(x0 <= x1 && y0 <= y1)) return false;  // between pixel centres
     if (!(x1 >= 0.0f && y1 >= 0.0f && x0 <= (float)(P.w - 1) && y0 <= (float)(P.h - 1))) return false;
     tx0 = (x0 <= 0.0f ? 0 : (int)x0) / GRAY_TILE;
     tx1 = (x1 >= (float)(P.w - 1) ? P.w - 1 : (int)x1) / GRAY_TILE;
@@ -1118,7 +1123,12 @@ __global__ __launch_bounds__(kBinThreads) void k_bin_pairs(TraceArgs A, const fl
     __shared__ int32_t s_full;
     if (*A.bin_mode != 0) return;
     const int t = threadIdx.x;
-    if (t < A.num_views) bin_proj(A.views[t], s_proj[t]);
+    if (t < A.num_views) {
+        bin_proj(A.views[t], s_proj[t]);
+        // half a pixel of jitter when on, and 0.05 pixel for the rounding of
+        // make_ray and of the projection (both near 1e-3 pixel)
+        s_proj[t].margin = (A.knobs.jitter && *A.knobs.jitter) ? 0.55f : 0.05f;
+    }
     for (int i = t; i < kBinHash; i += kBinThreads) s_key[i] = -1, s_cnt[i] = 0;
     if (t == 0) s_full = 0;
     __syncthreads();
