@@ -42,11 +42,14 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "train-step ms & rays/s at 1M Gaussians; p50/p99 frame latency vs plain launch"
-# the dominant unit: the collect pass, i.e. the per-tile walk (k_collect) and
-# the pooled task rounds that finish its walks (k_collect_pool, two launches)
-DOMINANT_KERNEL = "collect pass (k_collect + k_collect_pool)"
-PASS_KERNELS = ("k_collect<", "k_collect(", "k_collect_pool(")
-PASS_FIRST = ("k_collect<", "k_collect(")  # one launch per pass
+# the dominant unit: the collect pass, i.e. the tile bins (k_bin_setup, two
+# k_bin_pairs, k_bin_scan, k_bin_items) and the bins' tests (k_collect_binned);
+# the walk's kernels (k_collect, two k_collect_pool) are launched as the
+# device fallback and exit at once, and are counted with the pass
+DOMINANT_KERNEL = "collect pass (k_bin_* + k_collect_binned)"
+PASS_KERNELS = ("k_bin_setup(", "k_bin_pairs(", "k_bin_scan(", "k_bin_items(", "k_collect_binned(", "k_collect<",
+                "k_collect(", "k_collect_pool(")
+PASS_FIRST = ("k_bin_setup(",)  # one launch per pass
 # algorithmic bytes of the collect pass per Gaussian: the parameters its exact
 # hit tests depend on (mean 12, rotation 16, scale 12, opacity 4), read once;
 # hit lists, BVH nodes and the snapshot are implementation traffic
@@ -330,7 +333,8 @@ def run_train(args, traffic):
         stages.append(nm.forward_stage_ms())
     nm.set_device_timing(False)
     stage_ms = {k: sum(s[k] for s in stages) / len(stages) for k in stages[0]}
-    # the collect pass: the per-tile walk and the pooled task rounds
+    # the collect pass: the tile bins and their tests (and the walk's launches,
+    # which exit at once)
     launch_ms = stage_ms["collect"] + stage_ms["collect_tasks"]
     alg = COLLECT_BYTES_PER_GAUSSIAN * n
     achieved = alg / (launch_ms * 1e-3) / 1e9
